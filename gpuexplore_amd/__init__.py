@@ -1,0 +1,212 @@
+"""GPUexplore for MI355X: Python view of the C ABI in include/gpuexplore.h.
+
+The library (gpuexplore_amd/lib/libgpuexplore.so, built by
+``gpuexplore_amd.build``) does all the work; this module only marshals
+arguments with ctypes.  Nothing here falls back to a CPU implementation: if
+the library is missing, :func:`lib` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgpuexplore.so")
+CLI_PATH = os.path.join(HERE, "bin", "GPUexplore")
+
+GX_OK = 0
+ERRORS = {
+    -1: "NOFILE", -2: "FORMAT", -3: "RANGE", -4: "NOMEM", -5: "HIP", -6: "NODEVICE",
+    -7: "ARG", -8: "LIMIT", -9: "STATE", -10: "CAPACITY",
+}
+PROP_NONE, PROP_DEADLOCK, PROP_SAFETY, PROP_LIVENESS = 0, 1, 2, 3
+
+
+class GxError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {ERRORS.get(status, status)} ({status})")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("blocks", ctypes.c_uint32),
+        ("threads", ctypes.c_uint32),
+        ("kernel_iters", ctypes.c_uint32),
+        ("bucket_bytes", ctypes.c_uint32),
+        ("q_ints", ctypes.c_uint64),
+        ("property", ctypes.c_int32),
+        ("por", ctypes.c_int32),
+        ("cycle_proviso", ctypes.c_int32),
+        ("verbosity", ctypes.c_int32),
+        ("hash_seed", ctypes.c_uint64),
+        ("cand_capacity", ctypes.c_uint64),
+        ("frontier_capacity", ctypes.c_uint64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("states", ctypes.c_uint64),
+        ("transitions", ctypes.c_uint64),
+        ("levels", ctypes.c_uint64),
+        ("lookups", ctypes.c_uint64),
+        ("inserts", ctypes.c_uint64),
+        ("hash_launches", ctypes.c_uint64),
+        ("table_bytes", ctypes.c_uint64),
+        ("violation", ctypes.c_int32),
+        ("table_full", ctypes.c_int32),
+        ("t_explore_s", ctypes.c_double),
+        ("t_hash_s", ctypes.c_double),
+        ("t_expand_s", ctypes.c_double),
+        ("t_init_s", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "nr_procs", "bits_act", "proc_nrstates", "bits_statevector", "sv_nints", "liveness",
+        "max_buf_ints", "nr_trans", "state_bits", "key_words")]
+
+
+# exported symbols and their (restype, argtypes)
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "gx_model_load_gpf": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P)]),
+    "gx_model_load_gpf_text": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_P)]),
+    "gx_model_free": (None, [_P]),
+    "gx_model_info_get": (ctypes.c_int, [_P, ctypes.POINTER(ModelInfo)]),
+    "gx_options_default": (None, [ctypes.POINTER(Options)]),
+    "gx_explore": (ctypes.c_int, [_P, ctypes.POINTER(Options), ctypes.POINTER(Result)]),
+    "gx_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "gx_table_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                       ctypes.POINTER(_P)]),
+    "gx_table_free": (None, [_P]),
+    "gx_table_insert_host": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint64)]),
+    "gx_table_insert_device": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_float)]),
+    "gx_table_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "gx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "gx_device_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "gx_device_free": (ctypes.c_int, [_P]),
+    "gx_copy_to_device": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "gx_copy_to_host": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
+    "gx_bench_copy": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the in-tree libgpuexplore.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: run gpuexplore_amd.build.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != GX_OK:
+        raise GxError(rc, what)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib().gx_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Model:
+    """A loaded gpf model (gx_model handle)."""
+
+    def __init__(self, handle: int):
+        self.h = ctypes.c_void_p(handle)
+
+    @classmethod
+    def load(cls, path_noext: str, por: bool = False) -> "Model":
+        h = ctypes.c_void_p()
+        _check(lib().gx_model_load_gpf(path_noext.encode(), int(por), ctypes.byref(h)), "load " + path_noext)
+        return cls(h.value)
+
+    @classmethod
+    def from_text(cls, text: str, por: bool = False) -> "Model":
+        h = ctypes.c_void_p()
+        b = text.encode()
+        _check(lib().gx_model_load_gpf_text(b, len(b), int(por), ctypes.byref(h)), "load text")
+        return cls(h.value)
+
+    def info(self) -> dict:
+        i = ModelInfo()
+        _check(lib().gx_model_info_get(self.h, ctypes.byref(i)), "info")
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.gx_model_free(self.h)
+            self.h = None
+
+
+def options(**kw) -> Options:
+    o = Options()
+    lib().gx_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def explore(model: Model, **kw) -> dict:
+    """gx_explore with keyword options (field names of gx_options)."""
+    o = options(**kw)
+    r = Result()
+    rc = lib().gx_explore(model.h, ctypes.byref(o), ctypes.byref(r))
+    d = r.as_dict()
+    d["status"] = rc
+    if rc not in (GX_OK, -10):
+        raise GxError(rc, "explore")
+    return d
+
+
+class Table:
+    """Visited set used on its own (FIND-OR-PUT of 63-bit-chunk keys)."""
+
+    def __init__(self, q_ints: int, key_words: int, bucket_bytes: int = 128, seed: int = 1):
+        self.key_words = key_words
+        h = ctypes.c_void_p()
+        _check(lib().gx_table_create(q_ints, key_words, bucket_bytes, seed, ctypes.byref(h)), "table")
+        self.h = h
+
+    def insert(self, keys: Sequence[int]):
+        """keys: flat sequence of n*key_words ints (< 2^63).  Returns
+        (is_new bytes, inserted count, status)."""
+        import numpy as np
+        a = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+        n = a.size // self.key_words
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        ins = ctypes.c_uint64(0)
+        rc = lib().gx_table_insert_host(
+            self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ins))
+        return flags[:n], ins.value, rc
+
+    def count(self) -> int:
+        c = ctypes.c_uint64(0)
+        _check(lib().gx_table_count(self.h, ctypes.byref(c)), "count")
+        return c.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.gx_table_free(self.h)
+            self.h = None

@@ -1,0 +1,79 @@
+// gx_common.h -- structures shared by the host driver and the HIP kernels.
+//
+// Device state representation ("key"): a state vector of K = firstbit[N]
+// bits is stored as W = ceil(K/63) 64-bit words; word j carries bits
+// [63j, 63j+63) of the little-endian state bit string in bits 0..62 and has
+// bit 63 set.  A visited-set slot is the same W words (plus, with the cycle
+// proviso, a 32-bit level stamp appended after bit K), so a key is CASed into
+// a slot as is, and an all-zero word marks an empty slot.
+#pragma once
+#include <stdint.h>
+
+namespace gx {
+
+constexpr int kMaxProcs = 32;  // rules are N-bit masks packed in 32-bit ints (cu:205)
+constexpr int kMaxKeyWords = 5;  // 9 x 32 bits (cu:92) + 32-bit stamp fit in 5 x 63
+constexpr uint64_t kFlag = 1ull << 63;
+constexpr uint64_t kPayload = ~kFlag;
+constexpr int kHashFunctions = 8;  // buckets tried before the table is full (cu:88)
+
+// Compiled model: offsets (in 32-bit words) of the arrays inside one
+// contiguous buffer that a workgroup stages into LDS.
+struct ModelHdr {
+  uint32_t nprocs;      // N
+  uint32_t key_words;   // W
+  uint32_t state_bits;  // K
+  uint32_t nlab;        // labels with a rule table: 0 .. nlab-1
+  uint32_t total_words; // size of the buffer
+  uint32_t nrows;       // sum of rows over processes
+  // per process
+  uint32_t off_floc;    // field location: chunk | offset << 8 | width << 16
+  uint32_t off_rowbase; // first row of process p
+  uint32_t off_rowcnt;  // rows of process p (states with offsets)
+  // per row (p, s)
+  uint32_t off_loc_start;  // nrows + 1, CSR into loc_tgt
+  uint32_t off_syn_start;  // nrows + 1, CSR into syn_* (distinct labels, ascending)
+  uint32_t off_rowflags;   // nrows: bit0 = has a local entry, bit1 = bad offsets
+  uint32_t off_loc_tgt;    // decoded local target state ids
+  // per (row, label)
+  uint32_t off_syn_lab;
+  uint32_t off_syn_tstart; // into syn_tgt
+  uint32_t off_syn_tcnt;
+  uint32_t off_syn_tgt;    // concatenated targets T_p(a), decoded ids
+  // per label
+  uint32_t off_rule_start; // nlab + 1, CSR into rules
+  uint32_t off_rules;      // N-bit process masks
+};
+
+enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u };
+
+// Exploration modes of the successor kernel.
+enum ExpandMode : int { kModePlain = 0, kModePor = 1, kModePorProviso = 2 };
+
+// Device-resident control block of one exploration.  Kernels read the work
+// sizes from here, so BFS levels can be enqueued (or graph-replayed) without
+// a host round trip.
+struct Ctrl {
+  uint64_t front_count[2];  // frontier sizes; [cur] is being expanded
+  uint64_t cand_count;      // successors in the candidate buffer
+  uint64_t retry_count[2];  // frontier indices deferred when the candidate buffer filled
+  uint64_t lookups;         // FIND-OR-PUT operations issued
+  uint64_t inserts;         // new states
+  uint64_t transitions;     // successors generated
+  uint64_t expanded;        // states expanded
+  uint32_t cur;             // current frontier buffer
+  uint32_t retry_cur;       // retry list being consumed when use_retry
+  uint32_t use_retry;       // expand the retry list instead of the frontier
+  uint32_t level;           // BFS level being expanded
+  uint32_t violation;       // property violated at this level
+  uint32_t table_full;      // a key found no free slot in its kHashFunctions buckets
+  uint32_t overflow;        // frontier overflow (fatal)
+  uint32_t bad_state;       // a reached local state has no offsets (fatal)
+  uint32_t spin_timeout;    // a publish wait did not finish (fatal)
+  uint32_t stop;            // no further level runs
+  uint32_t done;            // frontier empty
+  uint32_t levels_done;     // levels completed
+  uint32_t pad[2];
+};
+
+}  // namespace gx

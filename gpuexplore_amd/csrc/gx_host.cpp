@@ -1,0 +1,499 @@
+// gx_host.cpp -- C ABI implementation: model handles, single-model
+// exploration driver, visited-set API and device helpers.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gpuexplore.h"
+#include "gx_kernels.h"
+#include "gx_model.h"
+
+using namespace gx;
+
+namespace {
+
+#define GX_TRY(expr)                      \
+  do {                                    \
+    hipError_t e_ = (expr);               \
+    if (e_ != hipSuccess) return map_hip(e_); \
+  } while (0)
+
+int map_hip(hipError_t e) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GX_E_NOMEM;
+  if (e == hipErrorNoDevice) return GX_E_NODEVICE;
+  return GX_E_HIP;
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// RAII device buffer
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    return hipMalloc(&p, bytes ? bytes : 8);
+  }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+struct Events {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  ~Events() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  hipEvent_t next() {
+    if (used == ev.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ev.push_back(e);
+    }
+    return ev[used++];
+  }
+};
+
+uint32_t round_threads(uint32_t t) {
+  if (t == 0) t = 256;
+  t = (t + 63) / 64 * 64;
+  return std::min<uint32_t>(512, std::max<uint32_t>(64, t));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" int gx_model_load_gpf_text(const char *text, size_t len, int por, gx_model **out) {
+  if (!text || !out) return GX_E_ARG;
+  gx_model *m = new (std::nothrow) gx_model();
+  if (!m) return GX_E_NOMEM;
+  m->por = por != 0;
+  int rc = parse_gpf(std::string(text, len), m->por, &m->gpf);
+  if (rc == GX_OK) rc = compile_model(m->gpf, &m->comp);
+  if (rc != GX_OK) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return GX_OK;
+}
+
+extern "C" int gx_model_load_gpf(const char *path_noext, int por, gx_model **out) {
+  if (!path_noext || !out) return GX_E_ARG;
+  std::string fn = std::string(path_noext) + ".gpf";  // cu:1814-1815
+  std::ifstream f(fn, std::ios::binary);
+  if (!f) return GX_E_NOFILE;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  std::string t = ss.str();
+  return gx_model_load_gpf_text(t.data(), t.size(), por, out);
+}
+
+extern "C" void gx_model_free(gx_model *m) { delete m; }
+
+extern "C" int gx_model_info_get(const gx_model *m, gx_model_info *o) {
+  if (!m || !o) return GX_E_ARG;
+  const Gpf &g = m->gpf;
+  o->nr_procs = g.nr_procs;
+  o->bits_act = g.bits_act;
+  o->proc_nrstates = g.proc_nrstates;
+  o->bits_statevector = g.bits_statevector;
+  o->sv_nints = g.sv_nints();
+  o->liveness = g.liveness;
+  o->max_buf_ints = g.max_buf_ints;
+  o->nr_trans = static_cast<uint32_t>(g.proc_trans.size());
+  o->state_bits = m->comp.hdr.state_bits;
+  o->key_words = m->comp.hdr.key_words;
+  return GX_OK;
+}
+
+extern "C" void gx_options_default(gx_options *o) {
+  memset(o, 0, sizeof *o);
+  o->kernel_iters = 1;
+  o->bucket_bytes = 128;
+  o->hash_seed = 0x5eed5eed5eedull;
+}
+
+extern "C" const char *gx_strerror(int s) {
+  switch (s) {
+    case GX_OK: return "ok";
+    case GX_E_NOFILE: return "input network does not exist";
+    case GX_E_FORMAT: return "malformed gpf input";
+    case GX_E_RANGE: return "gpf field out of range";
+    case GX_E_NOMEM: return "out of memory";
+    case GX_E_HIP: return "HIP runtime error";
+    case GX_E_NODEVICE: return "no HIP device detected";
+    case GX_E_ARG: return "invalid argument";
+    case GX_E_LIMIT: return "model exceeds a supported limit";
+    case GX_E_STATE: return "reached a local state without transition offsets";
+    case GX_E_CAPACITY: return "frontier or successor buffer exhausted";
+    default: return "unknown error";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-model exploration (cu:1989-2142, 2168-2172).
+extern "C" int gx_explore(const gx_model *m, const gx_options *opt, gx_result *res) {
+  if (!m || !opt || !res) return GX_E_ARG;
+  memset(res, 0, sizeof *res);
+  if (device_count() == 0) return GX_E_NODEVICE;
+  const auto t_init0 = std::chrono::steady_clock::now();
+  const ModelHdr &H = m->comp.hdr;
+  const int W = static_cast<int>(H.key_words);
+  const bool proviso = opt->por && opt->cycle_proviso;
+  const bool stamp = proviso;
+  const uint32_t K = H.state_bits;
+  const int WT = stamp ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
+  if (WT > kMaxKeyWords || W > kMaxKeyWords) return GX_E_LIMIT;
+  const uint32_t bb = opt->bucket_bytes ? opt->bucket_bytes : 128;
+  if (bb != 64 && bb != 128 && bb != 256) return GX_E_ARG;
+  const int BW = static_cast<int>(bb / 8);
+  if (BW / WT < 1) return GX_E_ARG;
+  const uint32_t threads = round_threads(opt->threads);
+  int dev = 0;
+  GX_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  GX_TRY(hipGetDeviceProperties(&prop, dev));
+  const uint32_t cus = static_cast<uint32_t>(prop.multiProcessorCount);
+  const uint32_t blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : cus * (2048 / threads);
+
+  size_t free_b = 0, total_b = 0;
+  GX_TRY(hipMemGetInfo(&free_b, &total_b));
+  // -q in 32-bit ints (cu:1838-1842); the default is a quarter of free
+  // memory, at most 8 GiB (the reference takes all of it, cu:2054-2066)
+  uint64_t table_bytes = opt->q_ints ? opt->q_ints * 4ull
+                                     : std::min<uint64_t>(free_b / 4, 8ull << 30);
+  uint64_t nbuckets = table_bytes / bb;
+  if (nbuckets == 0) return GX_E_ARG;
+  table_bytes = nbuckets * bb;
+  const uint64_t slots = nbuckets * static_cast<uint64_t>(BW / WT);
+
+  DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
+  GX_TRY(d_table.alloc(table_bytes));
+  GX_TRY(hipMemGetInfo(&free_b, &total_b));
+  // buffers from what is left: frontiers are bounded by the slot count
+  const uint64_t per_front = static_cast<uint64_t>(W) * 8ull + 8ull;  // key + retry index
+  uint64_t front_cap = opt->frontier_capacity;
+  if (!front_cap) front_cap = std::min<uint64_t>(slots, (free_b / 4) / (2 * per_front));
+  front_cap = std::max<uint64_t>(front_cap, 64);
+  uint64_t cand_cap = opt->cand_capacity;
+  if (!cand_cap)
+    cand_cap = std::min<uint64_t>(std::max<uint64_t>(4 * slots, 1u << 20), (free_b / 4) / (W * 8ull));
+  cand_cap = std::max<uint64_t>(cand_cap, 64);
+  GX_TRY(d_front0.alloc(front_cap * W * 8));
+  GX_TRY(d_front1.alloc(front_cap * W * 8));
+  GX_TRY(d_retry0.alloc(front_cap * 8));
+  GX_TRY(d_retry1.alloc(front_cap * 8));
+  GX_TRY(d_cand.alloc(cand_cap * W * 8));
+  GX_TRY(d_ctrl.alloc(sizeof(Ctrl)));
+  GX_TRY(d_model.alloc(m->comp.words.size() * 4));
+  GX_TRY(d_count.alloc(8));
+
+  hipStream_t st;
+  GX_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{st};
+
+  GX_TRY(hipMemcpyAsync(d_model.p, m->comp.words.data(), m->comp.words.size() * 4, hipMemcpyHostToDevice, st));
+  GX_TRY(hipMemsetAsync(d_table.p, 0, table_bytes, st));  // cu:656-663 init_queue
+  Ctrl c0{};
+  c0.cur = 1;
+  c0.level = 0xFFFFFFFFu;
+  c0.levels_done = 0xFFFFFFFFu;
+  c0.cand_count = 1;
+  GX_TRY(hipMemcpyAsync(d_ctrl.p, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  std::vector<uint64_t> s0(W, kFlag);  // all processes in local state 0 (cu:668-682)
+  GX_TRY(hipMemcpyAsync(d_cand.p, s0.data(), W * 8, hipMemcpyHostToDevice, st));
+
+  Table T{d_table.as<uint64_t>(), nbuckets, opt->hash_seed};
+  ExpandArgs ea{};
+  ea.model = d_model.as<uint32_t>();
+  ea.hdr = H;
+  ea.ctrl = d_ctrl.as<Ctrl>();
+  ea.front[0] = d_front0.as<uint64_t>();
+  ea.front[1] = d_front1.as<uint64_t>();
+  ea.cand = d_cand.as<uint64_t>();
+  ea.cand_cap = cand_cap;
+  ea.retry[0] = d_retry0.as<uint64_t>();
+  ea.retry[1] = d_retry1.as<uint64_t>();
+  ea.property = opt->property;
+  ea.bucket_words = BW;
+  ea.table = T;
+  InsertArgs ia{};
+  ia.ctrl = d_ctrl.as<Ctrl>();
+  ia.cand = d_cand.as<uint64_t>();
+  ia.front[0] = d_front0.as<uint64_t>();
+  ia.front[1] = d_front1.as<uint64_t>();
+  ia.front_cap = front_cap;
+  ia.table = T;
+  ia.state_bits = K;
+  ia.is_new = nullptr;
+  ia.append = 1;
+
+  const int mode = opt->por ? (proviso ? kModePorProviso : kModePor) : kModePlain;
+  const size_t lds_limit = 48 * 1024;
+  const bool lds = static_cast<size_t>(H.total_words) * 4 + expand_scratch_bytes(threads) <= lds_limit;
+  if (!lds) {
+    // scratch only
+  }
+
+  // initial state: FIND-OR-PUT of s0 into frontier 0 with stamp 0
+  GX_TRY(launch_insert(ia, W, WT, BW, stamp, 1, 64, st));
+  GX_TRY(launch_advance(ia.ctrl, st));
+  GX_TRY(hipStreamSynchronize(st));
+  const auto t_init1 = std::chrono::steady_clock::now();
+  res->t_init_s = std::chrono::duration<double>(t_init1 - t_init0).count();
+
+  const uint32_t kiters = std::max<uint32_t>(1, opt->kernel_iters);
+  Events evs;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ins_ev, exp_ev;
+  double hash_ms = 0, exp_ms = 0;
+  Ctrl ch{};
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    evs.used = 0;
+    ins_ev.clear();
+    exp_ev.clear();
+    for (uint32_t it = 0; it < kiters; it++) {
+      hipEvent_t a0 = evs.next(), a1 = evs.next(), b0 = evs.next(), b1 = evs.next();
+      if (!a0 || !a1 || !b0 || !b1) return GX_E_HIP;
+      GX_TRY(hipEventRecord(a0, st));
+      GX_TRY(launch_expand(ea, mode, W, WT, lds, blocks, threads, st));
+      GX_TRY(hipEventRecord(a1, st));
+      GX_TRY(hipEventRecord(b0, st));
+      GX_TRY(launch_insert(ia, W, WT, BW, stamp, blocks, threads, st));
+      GX_TRY(hipEventRecord(b1, st));
+      GX_TRY(launch_advance(ia.ctrl, st));
+      exp_ev.push_back({a0, a1});
+      ins_ev.push_back({b0, b1});
+    }
+    GX_TRY(hipMemcpyAsync(&ch, d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
+    GX_TRY(hipStreamSynchronize(st));
+    for (auto &e : exp_ev) {
+      float ms = 0;
+      GX_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+      exp_ms += ms;
+    }
+    for (auto &e : ins_ev) {
+      float ms = 0;
+      GX_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+      hash_ms += ms;
+    }
+    res->hash_launches += ins_ev.size();
+    if (opt->verbosity == 1) printf("%u\n", ch.levels_done);  // cu:2129-2131
+    if (ch.stop) break;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  res->t_explore_s = std::chrono::duration<double>(t1 - t0).count();
+  res->t_hash_s = hash_ms / 1e3;
+  res->t_expand_s = exp_ms / 1e3;
+
+  GX_TRY(hipMemsetAsync(d_count.p, 0, 8, st));
+  GX_TRY(launch_count(T, WT, BW, d_count.as<uint64_t>(), cus * 8, st));
+  uint64_t counted = 0;
+  GX_TRY(hipMemcpyAsync(&counted, d_count.p, 8, hipMemcpyDeviceToHost, st));
+  GX_TRY(hipStreamSynchronize(st));
+
+  res->states = counted;
+  res->transitions = ch.transitions;
+  res->levels = ch.level;  // levels completed
+  res->lookups = ch.lookups;
+  res->inserts = ch.inserts;
+  res->violation = ch.violation ? 1 : 0;
+  res->table_full = ch.table_full ? 1 : 0;
+  res->table_bytes = table_bytes;
+  if (ch.bad_state) return GX_E_STATE;
+  if (ch.overflow) return GX_E_CAPACITY;
+  if (ch.spin_timeout) return GX_E_HIP;
+  return GX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Visited-set API
+struct gx_table {
+  DevBuf words, ctrl, cand, status;
+  uint64_t nbuckets = 0, cand_cap = 0;
+  uint32_t W = 1, BW = 16;
+  uint64_t seed = 0;
+  hipStream_t st = nullptr;
+  ~gx_table() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t bucket_bytes,
+                               uint64_t hash_seed, gx_table **out) {
+  if (!out || key_words < 1 || key_words > static_cast<uint32_t>(kMaxKeyWords)) return GX_E_ARG;
+  if (bucket_bytes == 0) bucket_bytes = 128;
+  if (bucket_bytes != 64 && bucket_bytes != 128 && bucket_bytes != 256) return GX_E_ARG;
+  if (bucket_bytes / 8 < key_words) return GX_E_ARG;
+  if (device_count() == 0) return GX_E_NODEVICE;
+  gx_table *t = new (std::nothrow) gx_table();
+  if (!t) return GX_E_NOMEM;
+  t->W = key_words;
+  t->BW = bucket_bytes / 8;
+  t->seed = hash_seed;
+  t->nbuckets = q_ints * 4ull / bucket_bytes;
+  int rc = GX_OK;
+  if (t->nbuckets == 0) rc = GX_E_ARG;
+  if (rc == GX_OK && hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess) rc = GX_E_HIP;
+  if (rc == GX_OK && t->words.alloc(t->nbuckets * bucket_bytes) != hipSuccess) rc = GX_E_NOMEM;
+  if (rc == GX_OK && t->ctrl.alloc(sizeof(Ctrl)) != hipSuccess) rc = GX_E_NOMEM;
+  if (rc == GX_OK && hipMemsetAsync(t->words.p, 0, t->nbuckets * bucket_bytes, t->st) != hipSuccess) rc = GX_E_HIP;
+  if (rc == GX_OK && hipStreamSynchronize(t->st) != hipSuccess) rc = GX_E_HIP;
+  if (rc != GX_OK) {
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return GX_OK;
+}
+
+extern "C" void gx_table_free(gx_table *t) { delete t; }
+
+static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8_t *dev_status,
+                        uint64_t *inserted, float *kernel_ms) {
+  Ctrl c{};
+  c.cand_count = n;
+  GX_TRY(hipMemcpyAsync(t->ctrl.p, &c, sizeof c, hipMemcpyHostToDevice, t->st));
+  InsertArgs ia{};
+  ia.ctrl = t->ctrl.as<Ctrl>();
+  ia.cand = dev_keys;
+  ia.table = Table{t->words.as<uint64_t>(), t->nbuckets, t->seed};
+  ia.state_bits = 63 * t->W;
+  ia.is_new = dev_status;
+  ia.append = 0;
+  int dev = 0;
+  GX_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  GX_TRY(hipGetDeviceProperties(&prop, dev));
+  hipEvent_t e0, e1;
+  GX_TRY(hipEventCreate(&e0));
+  GX_TRY(hipEventCreate(&e1));
+  GX_TRY(hipEventRecord(e0, t->st));
+  hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, prop.multiProcessorCount * 8, 256, t->st);
+  GX_TRY(hipEventRecord(e1, t->st));
+  GX_TRY(hipMemcpyAsync(&c, t->ctrl.p, sizeof c, hipMemcpyDeviceToHost, t->st));
+  GX_TRY(hipStreamSynchronize(t->st));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  GX_TRY(le);
+  if (kernel_ms) *kernel_ms = ms;
+  if (inserted) *inserted = c.inserts;
+  if (c.spin_timeout) return GX_E_HIP;
+  if (c.table_full) return GX_E_CAPACITY;
+  return GX_OK;
+}
+
+extern "C" int gx_table_insert_device(gx_table *t, const uint64_t *dev_keys, uint64_t n,
+                                      uint64_t *inserted, float *kernel_ms) {
+  if (!t || (!dev_keys && n)) return GX_E_ARG;
+  return table_insert(t, dev_keys, n, nullptr, inserted, kernel_ms);
+}
+
+extern "C" int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
+                                    uint64_t *inserted) {
+  if (!t || (!keys && n)) return GX_E_ARG;
+  DevBuf k, s;
+  GX_TRY(k.alloc(n * t->W * 8));
+  GX_TRY(s.alloc(n));
+  std::vector<uint64_t> tmp(keys, keys + n * t->W);
+  for (auto &w : tmp) w |= kFlag;
+  GX_TRY(hipMemcpyAsync(k.p, tmp.data(), n * t->W * 8, hipMemcpyHostToDevice, t->st));
+  int rc = table_insert(t, k.as<uint64_t>(), n, s.as<uint8_t>(), inserted, nullptr);
+  if (is_new && n) {
+    GX_TRY(hipMemcpyAsync(is_new, s.p, n, hipMemcpyDeviceToHost, t->st));
+    GX_TRY(hipStreamSynchronize(t->st));
+  }
+  return rc;
+}
+
+extern "C" int gx_table_count(gx_table *t, uint64_t *count) {
+  if (!t || !count) return GX_E_ARG;
+  DevBuf d;
+  GX_TRY(d.alloc(8));
+  GX_TRY(hipMemsetAsync(d.p, 0, 8, t->st));
+  GX_TRY(launch_count(Table{t->words.as<uint64_t>(), t->nbuckets, t->seed}, t->W, t->BW,
+                      d.as<uint64_t>(), 1024, t->st));
+  GX_TRY(hipMemcpyAsync(count, d.p, 8, hipMemcpyDeviceToHost, t->st));
+  GX_TRY(hipStreamSynchronize(t->st));
+  return GX_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int gx_device_count(int *count) {
+  if (!count) return GX_E_ARG;
+  *count = device_count();
+  return GX_OK;
+}
+
+extern "C" int gx_device_alloc(uint64_t bytes, void **p) {
+  if (!p) return GX_E_ARG;
+  if (device_count() == 0) return GX_E_NODEVICE;
+  GX_TRY(hipMalloc(p, bytes ? bytes : 8));
+  return GX_OK;
+}
+
+extern "C" int gx_device_free(void *p) {
+  GX_TRY(hipFree(p));
+  return GX_OK;
+}
+
+extern "C" int gx_copy_to_device(void *dst, const void *src, uint64_t bytes) {
+  GX_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GX_OK;
+}
+
+extern "C" int gx_copy_to_host(void *dst, const void *src, uint64_t bytes) {
+  GX_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GX_OK;
+}
+
+extern "C" int gx_bench_copy(uint64_t bytes, int reps, double *gbps) {
+  if (!gbps || bytes < 16 || reps < 1) return GX_E_ARG;
+  if (device_count() == 0) return GX_E_NODEVICE;
+  const uint64_t n = bytes / 16;
+  DevBuf a, b;
+  GX_TRY(a.alloc(n * 16));
+  GX_TRY(b.alloc(n * 16));
+  GX_TRY(hipMemset(a.p, 1, n * 16));
+  int dev = 0;
+  GX_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  GX_TRY(hipGetDeviceProperties(&prop, dev));
+  const uint32_t blocks = prop.multiProcessorCount * 16;
+  hipEvent_t e0, e1;
+  GX_TRY(hipEventCreate(&e0));
+  GX_TRY(hipEventCreate(&e1));
+  GX_TRY(launch_copy(a.as<uint4>(), b.as<uint4>(), n, blocks, nullptr));  // warm-up
+  GX_TRY(hipEventRecord(e0, nullptr));
+  for (int i = 0; i < reps; i++) GX_TRY(launch_copy(a.as<uint4>(), b.as<uint4>(), n, blocks, nullptr));
+  GX_TRY(hipEventRecord(e1, nullptr));
+  GX_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  GX_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbps = 2.0 * static_cast<double>(n * 16) * reps / (ms / 1e3) / 1e9;
+  return GX_OK;
+}

@@ -1,0 +1,580 @@
+// gx_kernels.hip -- gather/explore kernels for gfx950.
+//
+// One BFS level is three launches on one stream:
+//   k_expand   frontier -> candidate successors   (cu:870-1151 gather,
+//              cu:1306-1696 gather_por)
+//   k_insert   candidates -> visited set + next frontier (cu:361-450
+//              FINDORPUT_WARP, cu:707-735 publish, cu:804-869 claim)
+//   k_advance  level bookkeeping and termination (cu:858-868, 1164-1178,
+//              2109-2127)
+// All sizes are read from the device control block, so levels can be
+// enqueued back to back without host round trips.
+#include "gx_kernels.h"
+
+namespace gx {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------------------
+// Successor generation.
+//
+// A wave of 64 lanes expands G = floor(64/N) states at once: lane
+// (g*N + p) handles process p of the g-th state (the reference uses groups
+// of N lanes inside a 32-lane warp, cu:64-82).  Synchronising labels are
+// walked in ascending order; the enabled set En(a) is a ballot over the
+// group; rules of the label are split over the group's lanes and each lane
+// enumerates the product of target lists of its enabled rules, lowest
+// process varying fastest (cu:951-1130).
+//
+// Every state is walked in a counting pass and again in an emitting pass:
+// the counts give each lane its output offset (wave prefix sum) and the
+// wave reserves its whole output with one atomic.  A wave whose output does
+// not fit in the candidate buffer defers its states to a retry list.
+template <int W, bool LDS, int MODE, int WT>
+__global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
+  extern __shared__ uint32_t smem[];
+  const ModelHdr &H = a.hdr;
+  Ctrl *C = a.ctrl;
+  if (LDS) {
+    for (uint32_t i = threadIdx.x; i < H.total_words; i += blockDim.x) smem[i] = a.model[i];
+  }
+  const uint32_t *M = LDS ? smem : a.model;
+  const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
+  const int wib = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  uint2 *xch = reinterpret_cast<uint2 *>(smem + xoff) + wib * 64;
+  uint32_t *xc = smem + xoff + wpb * 128 + wib * 64;
+  uint32_t *xf = smem + xoff + wpb * 192 + wib * 64;
+  __syncthreads();
+  if (C->stop) return;
+
+  const uint32_t cur = C->cur;
+  const uint32_t use_retry = C->use_retry;
+  const uint32_t rc = C->retry_cur;
+  const uint64_t nwork = use_retry ? C->retry_count[rc] : C->front_count[cur];
+  const uint64_t *retry_in = a.retry[rc];
+  uint64_t *retry_out = a.retry[rc ^ 1u];
+  uint64_t *retry_out_count = &C->retry_count[rc ^ 1u];
+  const uint64_t *front = a.front[cur];
+  const uint32_t level = C->level;
+
+  const int N = static_cast<int>(H.nprocs);
+  const int G = 64 / N;
+  const int lane = lane_id();
+  const int g = lane / N;
+  const int p = lane - g * N;
+  const bool in_group = g < G;
+  const int base = in_group ? g * N : 0;
+  const uint64_t nmask = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
+
+  const uint32_t floc = in_group ? M[H.off_floc + p] : 0u;
+  const uint32_t rowbase = in_group ? M[H.off_rowbase + p] : 0u;
+  const uint32_t rowcnt = in_group ? M[H.off_rowcnt + p] : 0u;
+  const uint32_t fwidth = floc >> 16;
+  const uint32_t fmask = (fwidth >= 32) ? 0xFFFFFFFFu : ((1u << fwidth) - 1u);
+  const bool last_proc = in_group && p == N - 1;
+  const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
+  const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
+  const bool safety = a.property == 2;
+
+  SlotFmt<WT> fmt;
+  make_fmt<WT, MODE == kModePorProviso>(fmt, H.state_bits, 0u);
+
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint64_t acc_trans = 0, acc_states = 0;
+  bool viol = false, bad = false;
+
+  for (uint64_t b0 = gw * G; b0 < nwork; b0 += nw * G) {
+    const uint64_t si = b0 + g;
+    const bool has = in_group && si < nwork;
+    uint64_t fi = 0;
+    if (has) fi = use_retry ? retry_in[si] : si;
+    uint64_t key[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key[j] = has ? front[fi * W + j] : kFlag;
+    uint32_t l0 = 0, l1 = 0, e0 = 0, e1 = 0, rf = 0;
+    if (has) {
+      const uint32_t sp = kget<W>(key, floc);
+      if (sp < rowcnt) {
+        const uint32_t row = rowbase + sp;
+        rf = M[H.off_rowflags + row];
+        if (rf & kRowBad) {
+          bad = true;
+        } else {
+          l0 = M[H.off_loc_start + row];
+          l1 = M[H.off_loc_start + row + 1];
+          e0 = M[H.off_syn_start + row];
+          e1 = M[H.off_syn_start + row + 1];
+        }
+      } else {
+        bad = true;
+      }
+    }
+
+    // Walk the synchronising labels of the G states in ascending order.
+    // on_label(cur, en, mine) runs on every lane of a group whose label is
+    // `cur`; on_rule(r, cur) on the lane owning an enabled rule r.
+    auto walk = [&](auto &&on_label, auto &&on_rule) {
+      uint32_t e = e0;
+      for (;;) {
+        const uint32_t mylab = (e < e1) ? M[H.off_syn_lab + e] : kNone;
+        const uint32_t curl = group_min(mylab, N, base, p, in_group);
+        if (!ballot(in_group && curl != kNone)) break;
+        const bool mine = in_group && mylab == curl && curl != kNone;
+        const uint64_t en = (ballot(mine) >> base) & nmask;
+        xch[lane] = mine ? make_uint2(M[H.off_syn_tstart + e], M[H.off_syn_tcnt + e]) : make_uint2(0u, 0u);
+        wave_sync();
+        if (in_group && curl != kNone) {
+          on_label(curl, en, mine);
+          const uint32_t rs = M[H.off_rule_start + curl], re = M[H.off_rule_start + curl + 1];
+          for (uint32_t j = rs + static_cast<uint32_t>(p); j < re; j += static_cast<uint32_t>(N)) {
+            const uint32_t r = M[H.off_rules + j];
+            if ((static_cast<uint64_t>(r) & ~en) == 0) on_rule(r, curl);
+          }
+        }
+        wave_sync();
+        if (mine) e++;
+      }
+    };
+    auto rule_product = [&](uint32_t r) {
+      uint64_t prod = 1;
+      for (uint32_t rr = r; rr; rr &= rr - 1) prod *= xch[base + __ffs(rr) - 1].y;
+      return prod;
+    };
+    // does T_{N-1}(cur) contain local state 1 (successor violates safety)
+    auto last_hits_one = [&]() {
+      const uint2 t = xch[base + N - 1];
+      for (uint32_t i = 0; i < t.y; i++)
+        if ((M[H.off_syn_tgt + t.x + i] & fmask_last) == 1u) return true;
+      return false;
+    };
+
+    // ---------------- pass 1: enabledness, safety, (POR) clusters
+    bool any_enabled = false;
+    uint32_t cluster0 = in_group ? (1u << p) : 0u;
+    bool in_en = false;
+    uint32_t rnew = 0;  // enabled rules with a new successor (proviso)
+    bool local_new = false;
+    if (safety && last_proc) {
+      for (uint32_t t = l0; t < l1; t++)
+        if ((M[H.off_loc_tgt + t] & fmask) == 1u) viol = true;
+    }
+    walk(
+        [&](uint32_t curl, uint64_t en, bool mine) {
+          if (MODE != kModePlain && mine) {
+            const uint32_t rs = M[H.off_rule_start + curl], re = M[H.off_rule_start + curl + 1];
+            for (uint32_t j = rs; j < re; j++) {
+              const uint32_t r = M[H.off_rules + j];
+              if ((r >> p) & 1u) {
+                cluster0 |= r;
+                if ((static_cast<uint64_t>(r) & ~en) == 0 && rule_product(r) > 0) in_en = true;
+              }
+            }
+          }
+        },
+        [&](uint32_t r, uint32_t curl) {
+          any_enabled = true;
+          if (safety && ((r >> (N - 1)) & 1u) && last_hits_one()) viol = true;
+          if (MODE == kModePorProviso) {
+            // enumerate until a successor is new (not visited at level <= L)
+            const uint64_t prod = rule_product(r);
+            bool found_new = false;
+            for (uint64_t k = 0; k < prod && !found_new; k++) {
+              uint64_t s[W];
+#pragma unroll
+              for (int j = 0; j < W; j++) s[j] = key[j];
+              uint64_t rem = k;
+              for (uint32_t rr = r; rr; rr &= rr - 1) {
+                const int q = __ffs(rr) - 1;
+                const uint2 t = xch[base + q];
+                const uint32_t d = static_cast<uint32_t>(rem % t.y);
+                rem /= t.y;
+                kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
+              }
+              uint64_t v[WT];
+              slot_value<W, WT>(s, fmt, v);
+              uint32_t stamp = 0;
+              const int f = find_thread<WT>(a.table, v, fmt, key_hash<W>(s, a.table.seed),
+                                            a.bucket_words, H.state_bits, &stamp);
+              if (!f || stamp > level) found_new = true;
+            }
+            if (found_new) rnew |= r;
+          }
+          (void)curl;
+        });
+    const bool local_entry = (rf & kRowHasLocal) != 0;
+    const bool dead = has && ((ballot(local_entry) >> base) & nmask) == 0 &&
+                      ((ballot(any_enabled) >> base) & nmask) == 0;
+    if (a.property == 1 && dead) viol = true;
+
+    // ---------------- POR: cluster selection (cu:1633-1672)
+    uint32_t csel = static_cast<uint32_t>(nmask);
+    if (MODE != kModePlain) {
+      bool prov;
+      if (MODE == kModePor) {
+        prov = (l1 > l0) || in_en;
+      } else {
+        for (uint32_t t = l0; t < l1 && !local_new; t++) {
+          uint64_t s[W];
+#pragma unroll
+          for (int j = 0; j < W; j++) s[j] = key[j];
+          kset<W>(s, floc, M[H.off_loc_tgt + t]);
+          uint64_t v[WT];
+          slot_value<W, WT>(s, fmt, v);
+          uint32_t stamp = 0;
+          const int f = find_thread<WT>(a.table, v, fmt, key_hash<W>(s, a.table.seed), a.bucket_words,
+                                        H.state_bits, &stamp);
+          if (!f || stamp > level) local_new = true;
+        }
+        // OR of the new-successor rules over the group
+        uint32_t acc = rnew;
+        for (int j = 1; j < N; j <<= 1) {
+          const int src = in_group ? base + ((p + j) % N) : lane;
+          acc |= __shfl(acc, src, 64);
+        }
+        prov = local_new || ((acc >> p) & 1u);
+      }
+      const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
+      xc[lane] = c;
+      wave_sync();
+      uint32_t K = c, done = 0;
+      bool ok = false;
+      if (in_group) {
+        while (K & ~done) {  // closure (cu:1641-1649)
+          const int i = __ffs(K & ~done) - 1;
+          done |= 1u << i;
+          const uint32_t ci = xc[base + i];
+          if ((ci >> i) & 1u) ok = true;
+          K |= ci;
+        }
+      }
+      xf[lane] = ok ? K : 0u;  // cu:1651-1655
+      wave_sync();
+      if (in_group) {
+        int best = N;
+        for (int q = 0; q < N; q++) {  // first smallest non-empty cluster (cu:1657-1666)
+          const uint32_t fq = xf[base + q];
+          const int pc = __popc(fq);
+          if (fq && pc < best) {
+            best = pc;
+            csel = fq;
+          }
+        }
+      }
+      wave_sync();
+    }
+
+    // ---------------- pass 2: count (ample) successors
+    uint64_t cnt = 0;
+    if (has && ((csel >> p) & 1u)) cnt += l1 - l0;
+    walk([&](uint32_t, uint64_t, bool) {},
+         [&](uint32_t r, uint32_t) {
+           if (has && (r & ~csel) == 0) cnt += rule_product(r);
+         });
+    uint64_t total = 0;
+    const uint64_t off = wave_excl_scan(cnt, &total);
+    uint64_t obase = 0;
+    int ok = 1;
+    if (total) {
+      if (lane == 0) {
+        if (total > a.cand_cap) {
+          ok = -1;
+        } else {
+          uint64_t old = __hip_atomic_load(&C->cand_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          for (;;) {
+            if (old + total > a.cand_cap) {
+              ok = 0;
+              break;
+            }
+            const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long *>(&C->cand_count),
+                                            static_cast<unsigned long long>(old),
+                                            static_cast<unsigned long long>(old + total));
+            if (prev == old) {
+              obase = old;
+              break;
+            }
+            old = prev;
+          }
+        }
+      }
+      ok = __shfl(ok, 0, 64);
+      obase = __shfl(obase, 0, 64);
+    }
+    if (ok <= 0) {
+      // defer this wave's states to the retry list
+      if (ok < 0 && lane == 0) C->overflow = 2;
+      const bool lead = has && p == 0;
+      const uint64_t m = ballot(lead);
+      uint64_t rb = 0;
+      if (lane == 0) rb = atomicAdd(reinterpret_cast<unsigned long long *>(retry_out_count),
+                                    static_cast<unsigned long long>(__popcll(m)));
+      rb = __shfl(rb, 0, 64);
+      if (lead) retry_out[rb + __popcll(m & ((1ull << lane) - 1ull))] = fi;
+      continue;
+    }
+    acc_trans += total;
+    acc_states += __popcll(ballot(has && p == 0));
+
+    // ---------------- pass 3: emit
+    uint64_t pos = obase + off;
+    if (has && ((csel >> p) & 1u)) {
+      for (uint32_t t = l0; t < l1; t++) {
+        uint64_t s[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) s[j] = key[j];
+        kset<W>(s, floc, M[H.off_loc_tgt + t]);
+#pragma unroll
+        for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
+        pos++;
+      }
+    }
+    walk([&](uint32_t, uint64_t, bool) {},
+         [&](uint32_t r, uint32_t) {
+           if (!has || (r & ~csel) != 0) return;
+           const uint64_t prod = rule_product(r);
+           for (uint64_t k = 0; k < prod; k++) {
+             uint64_t s[W];
+#pragma unroll
+             for (int j = 0; j < W; j++) s[j] = key[j];
+             uint64_t rem = k;
+             for (uint32_t rr = r; rr; rr &= rr - 1) {
+               const int q = __ffs(rr) - 1;
+               const uint2 t = xch[base + q];
+               const uint32_t d = static_cast<uint32_t>(rem % t.y);
+               rem /= t.y;
+               kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
+             }
+#pragma unroll
+             for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
+             pos++;
+           }
+         });
+  }
+  if (lane == 0) {
+    if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
+                             static_cast<unsigned long long>(acc_trans));
+    if (acc_states) atomicAdd(reinterpret_cast<unsigned long long *>(&C->expanded),
+                              static_cast<unsigned long long>(acc_states));
+  }
+  if (ballot(viol) && lane == 0) C->violation = 1;
+  if (ballot(bad) && lane == 0) C->bad_state = 1;
+}
+
+// ---------------------------------------------------------------------------
+// FIND-OR-PUT over the candidate buffer; new states are appended to the
+// next frontier with one atomic per wave and a ballot rank.
+template <int W, int WT, int BW, bool STAMP>
+__global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
+  Ctrl *C = a.ctrl;
+  if (C->stop) return;
+  const uint64_t n = C->cand_count;
+  const uint32_t nxt = C->cur ^ 1u;
+  uint64_t *out = a.front[nxt];
+  unsigned long long *out_count = reinterpret_cast<unsigned long long *>(&C->front_count[nxt]);
+  SlotFmt<WT> fmt;
+  make_fmt<WT, STAMP>(fmt, a.state_bits, C->level + 1u);
+  constexpr int LPB = BW / 2;
+  constexpr int CPW = 64 / LPB;
+  const int lane = lane_id();
+  const int sub = lane % LPB;
+  const int cg = lane / LPB;
+  const int gbase = cg * LPB;
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint64_t acc_new = 0;
+  bool full = false, timeout = false, overflow = false;
+  for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
+    const uint64_t ci = c0 + cg;
+    const bool active = ci < n;
+    uint64_t key[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
+    uint64_t v[WT];
+    slot_value<W, WT>(key, fmt, v);
+    const uint64_t h = key_hash<W>(key, a.table.seed);
+    const int st = find_or_put_group<WT, BW>(a.table, v, fmt, h, sub, gbase, active);
+    const bool is_new = active && st == kInserted;
+    full |= active && st == kFull;
+    timeout |= active && st == kTimeout;
+    if (a.is_new && active && sub == 0) a.is_new[ci] = is_new ? 1 : 0;
+    const uint64_t m = ballot(is_new && sub == 0);
+    if (m && !a.append) {
+      acc_new += __popcll(m);
+    } else if (m) {
+      const uint32_t cnt = __popcll(m);
+      uint64_t ob = 0;
+      if (lane == 0) ob = atomicAdd(out_count, static_cast<unsigned long long>(cnt));
+      ob = __shfl(ob, 0, 64);
+      if (is_new) {
+        const uint64_t idx = ob + __popcll(m & ((1ull << gbase) - 1ull));
+        if (idx < a.front_cap) {
+#pragma unroll
+          for (int j = 0; j < W; j++)
+            if (j % LPB == sub) out[idx * W + j] = key[j];
+        } else {
+          overflow = true;
+        }
+      }
+      acc_new += cnt;
+    }
+  }
+  if (lane == 0 && acc_new)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&C->inserts), static_cast<unsigned long long>(acc_new));
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(n));
+  if (ballot(full) && lane == 0) C->table_full = 1;
+  if (ballot(timeout) && lane == 0) C->spin_timeout = 1;
+  if (ballot(overflow) && lane == 0) C->overflow = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Level bookkeeping (one thread).  A level is complete when no states were
+// deferred; then the frontiers swap.  Exploration stops after a level with a
+// violation (level-granular early stop), on a full table or a fatal error,
+// or when the next frontier is empty.
+__global__ void k_advance(Ctrl *c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->stop) return;
+  c->cand_count = 0;
+  if (c->overflow || c->bad_state || c->spin_timeout || c->table_full) {
+    c->stop = 1;
+    return;
+  }
+  const uint32_t rc = c->retry_cur;
+  if (c->retry_count[rc ^ 1u] > 0) {
+    c->retry_count[rc] = 0;
+    c->retry_cur = rc ^ 1u;
+    c->use_retry = 1;
+    return;
+  }
+  c->retry_count[rc] = 0;
+  c->use_retry = 0;
+  c->front_count[c->cur] = 0;
+  c->cur ^= 1u;
+  c->level += 1u;
+  c->levels_done += 1u;
+  if (c->violation) c->stop = 1;
+  if (c->front_count[c->cur] == 0) {
+    c->done = 1;
+    c->stop = 1;
+  }
+}
+
+// occupied slots (cu:687-704)
+__global__ void k_count(Table t, int WT, int BW, unsigned long long *out) {
+  __shared__ unsigned long long acc;
+  if (threadIdx.x == 0) acc = 0;
+  __syncthreads();
+  const int SPB = BW / WT;
+  unsigned long long mine = 0;
+  const uint64_t nslots = t.nbuckets * static_cast<uint64_t>(SPB);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nslots;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t b = i / SPB, s = i % SPB;
+    mine += t.words[b * BW + s * WT] != 0;
+  }
+  atomicAdd(&acc, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, acc);
+}
+
+// ---------------------------------------------------------------------------
+size_t expand_scratch_bytes(uint32_t threads) {
+  // per wave: 64 x uint2 exchange + 64 x uint32 clusters + 64 x uint32 finals
+  return static_cast<size_t>(threads / 64) * 64 * (8 + 4 + 4) + 16;
+}
+
+template <int W, int WT>
+static hipError_t expand_w(const ExpandArgs &a, int mode, bool lds, uint32_t blocks,
+                           uint32_t threads, hipStream_t s) {
+  const size_t shm = (lds ? ((a.hdr.total_words + 3u) & ~3u) * 4u : 0u) + expand_scratch_bytes(threads);
+  if (mode == kModePlain) {
+    if (lds)
+      hipLaunchKernelGGL((k_expand<W, true, kModePlain, W>), dim3(blocks), dim3(threads), shm, s, a);
+    else
+      hipLaunchKernelGGL((k_expand<W, false, kModePlain, W>), dim3(blocks), dim3(threads), shm, s, a);
+  } else if (mode == kModePor) {
+    if (lds)
+      hipLaunchKernelGGL((k_expand<W, true, kModePor, W>), dim3(blocks), dim3(threads), shm, s, a);
+    else
+      hipLaunchKernelGGL((k_expand<W, false, kModePor, W>), dim3(blocks), dim3(threads), shm, s, a);
+  } else {
+    if (lds)
+      hipLaunchKernelGGL((k_expand<W, true, kModePorProviso, WT>), dim3(blocks), dim3(threads), shm, s, a);
+    else
+      hipLaunchKernelGGL((k_expand<W, false, kModePorProviso, WT>), dim3(blocks), dim3(threads), shm, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_expand(const ExpandArgs &a, int mode, int key_words, int slot_words, bool lds,
+                         uint32_t blocks, uint32_t threads, hipStream_t s) {
+  const bool plus = slot_words == key_words + 1;
+  if (slot_words != key_words && !plus) return hipErrorInvalidValue;
+  switch (key_words) {
+    case 1: return plus ? expand_w<1, 2>(a, mode, lds, blocks, threads, s) : expand_w<1, 1>(a, mode, lds, blocks, threads, s);
+    case 2: return plus ? expand_w<2, 3>(a, mode, lds, blocks, threads, s) : expand_w<2, 2>(a, mode, lds, blocks, threads, s);
+    case 3: return plus ? expand_w<3, 4>(a, mode, lds, blocks, threads, s) : expand_w<3, 3>(a, mode, lds, blocks, threads, s);
+    case 4: return plus ? expand_w<4, 5>(a, mode, lds, blocks, threads, s) : expand_w<4, 4>(a, mode, lds, blocks, threads, s);
+    case 5: return plus ? hipErrorInvalidValue : expand_w<5, 5>(a, mode, lds, blocks, threads, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int W, int WT, bool STAMP>
+static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  switch (bw) {
+    case 8:
+      if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break; }
+      return hipErrorInvalidValue;
+    case 16: hipLaunchKernelGGL((k_insert<W, WT, 16, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_insert<W, WT, 32, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_insert(const InsertArgs &a, int key_words, int slot_words, int bucket_words,
+                         bool stamp, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  const bool plus = slot_words == key_words + 1;
+  if (!stamp && plus) return hipErrorInvalidValue;
+  if (slot_words != key_words && !plus) return hipErrorInvalidValue;
+#define GX_INS(Wk, WTk)                                                            \
+  return stamp ? insert_bw<Wk, WTk, true>(a, bucket_words, blocks, threads, s)     \
+               : insert_bw<Wk, WTk, false>(a, bucket_words, blocks, threads, s)
+  switch (key_words) {
+    case 1: if (plus) { GX_INS(1, 2); } else { GX_INS(1, 1); }
+    case 2: if (plus) { GX_INS(2, 3); } else { GX_INS(2, 2); }
+    case 3: if (plus) { GX_INS(3, 4); } else { GX_INS(3, 3); }
+    case 4: if (plus) { GX_INS(4, 5); } else { GX_INS(4, 4); }
+    case 5: if (plus) return hipErrorInvalidValue; else { GX_INS(5, 5); }
+    default: return hipErrorInvalidValue;
+  }
+#undef GX_INS
+}
+
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(Ctrl *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const Table &t, int slot_words, int bucket_words, uint64_t *dev_count,
+                        uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_count, dim3(blocks), dim3(256), 0, s, t, slot_words, bucket_words,
+                     reinterpret_cast<unsigned long long *>(dev_count));
+  return hipGetLastError();
+}
+
+}  // namespace gx

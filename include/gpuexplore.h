@@ -1,0 +1,155 @@
+/*
+ * gpuexplore.h -- C ABI of the MI355X GPUexplore library (libgpuexplore.so).
+ *
+ * The reference (GPUexplore/src/GPUexplore.cu, cited as cu:N) has no library
+ * interface: its boundary is the `GPUexplore <model> [options]` process
+ * (argv, <model>.gpf, stdout/stderr, exit code) and, inside it, the loader,
+ * device setup, BFS driver and count steps of main().  Each entry point
+ * below replaces one of those steps, so the CLI (gpuexplore_amd/csrc/
+ * gx_cli.cpp) and any other host can be rebuilt on top of it.
+ *
+ * Conventions: every function returns 0 or a negative GX_E* status and never
+ * prints or exits; the caller owns option and result structs; a model handle
+ * owns its host arrays; explorations allocate and free their device buffers
+ * themselves.  Distinct handles may be used from distinct host threads.
+ */
+#ifndef GPUEXPLORE_H
+#define GPUEXPLORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GX_OK 0
+#define GX_E_NOFILE (-1)    /* cu:1984-1986 "input network does not exist" */
+#define GX_E_FORMAT (-2)    /* truncated / malformed gpf */
+#define GX_E_RANGE (-3)     /* field widths, offsets or labels out of range */
+#define GX_E_NOMEM (-4)     /* host or device allocation failed */
+#define GX_E_HIP (-5)       /* HIP runtime error */
+#define GX_E_NODEVICE (-6)  /* cu:2000-2003 "No CUDA compatible GPU detected" */
+#define GX_E_ARG (-7)       /* invalid option or argument */
+#define GX_E_LIMIT (-8)     /* model exceeds a supported limit */
+#define GX_E_STATE (-9)     /* a reached local state has no offsets in the gpf */
+#define GX_E_CAPACITY (-10) /* frontier or candidate buffer exhausted */
+
+/* properties (cu:27) */
+#define GX_PROP_NONE 0
+#define GX_PROP_DEADLOCK 1
+#define GX_PROP_SAFETY 2
+#define GX_PROP_LIVENESS 3
+
+typedef struct gx_model gx_model;
+
+/* Parse <path_noext>.gpf (cu:1814-1815, 1885-1983).  `por` adds the extra
+ * state-vector bit the reference loader adds (cu:1912). */
+int gx_model_load_gpf(const char *path_noext, int por, gx_model **out);
+
+/* Same, from gpf text in memory. */
+int gx_model_load_gpf_text(const char *text, size_t len, int por, gx_model **out);
+
+void gx_model_free(gx_model *m);
+
+typedef struct gx_model_info {
+  uint32_t nr_procs;          /* cu:1899-1902 */
+  uint32_t bits_act;          /* cu:1903-1906 */
+  uint32_t proc_nrstates;     /* cu:1907-1910 */
+  uint32_t bits_statevector;  /* cu:1911-1914, +1 under POR */
+  uint32_t sv_nints;          /* cu:1923 */
+  uint32_t liveness;          /* gpf line 1, cu:1888-1898 */
+  uint32_t max_buf_ints;      /* cu:1935-1938 */
+  uint32_t nr_trans;          /* cu:1951-1954 */
+  uint32_t state_bits;        /* firstbit[N]: bits used by process fields */
+  uint32_t key_words;         /* 64-bit words of the device state key */
+} gx_model_info;
+
+int gx_model_info_get(const gx_model *m, gx_model_info *out);
+
+/* Exploration options: the reference's -b -t -k -q -d/-p --por
+ * --cycle-proviso (cu:1785-1883) as orthogonal fields.  The reference's
+ * flag coupling (-d forces the proviso off, -p forces it on) is applied by
+ * the CLI, not here. */
+typedef struct gx_options {
+  uint32_t blocks;           /* -b: workgroups per launch (0 = auto) */
+  uint32_t threads;          /* -t: threads per workgroup, multiple of 64 (0 = auto) */
+  uint32_t kernel_iters;     /* -k: BFS levels enqueued per host sync (0 = 1) */
+  uint32_t bucket_bytes;     /* hash bucket width: 64, 128 or 256 (0 = 128) */
+  uint64_t q_ints;           /* -q: hash table size in 32-bit ints (0 = auto) */
+  int32_t property;          /* GX_PROP_* */
+  int32_t por;               /* --por */
+  int32_t cycle_proviso;     /* --cycle-proviso */
+  int32_t verbosity;         /* -v (only level 0/1 affect this call: progress counters) */
+  uint64_t hash_seed;        /* seeds the bucket hash; counts do not depend on it */
+  uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
+  uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
+} gx_options;
+
+void gx_options_default(gx_options *o);
+
+typedef struct gx_result {
+  uint64_t states;        /* distinct states in the visited set (cu:2168-2172) */
+  uint64_t transitions;   /* successors generated (under POR: ample ones) */
+  uint64_t levels;        /* BFS levels expanded */
+  uint64_t lookups;       /* candidates offered to the global table (FIND-OR-PUT) */
+  uint64_t inserts;       /* candidates that were new */
+  uint64_t hash_launches; /* FIND-OR-PUT kernel launches */
+  uint64_t table_bytes;   /* bytes of the visited set */
+  int32_t violation;      /* deadlock / safety violation found (cu:2150-2162) */
+  int32_t table_full;     /* cu:2164-2166 "problem with hash table" */
+  double t_explore_s;     /* wall time of the exploration (excludes parse/alloc/init) */
+  double t_hash_s;        /* summed FIND-OR-PUT kernel time (HIP events) */
+  double t_expand_s;      /* summed successor-generation kernel time (HIP events) */
+  double t_init_s;        /* device allocation + table initialisation */
+} gx_result;
+
+/* Explore the state space on the current HIP device: replaces the device
+ * setup, BFS driver loop and count of main() (cu:1989-2142, 2168-2172). */
+int gx_explore(const gx_model *m, const gx_options *o, gx_result *r);
+
+const char *gx_strerror(int status);
+
+/* ------------------------------------------------------------------ */
+/* Visited-set access (FIND-OR-PUT on its own), the hash-insert phase of the
+ * exploration (cu:361-450 FINDORPUT_WARP, cu:656-704 init/count).  Keys are
+ * `key_words` 64-bit words whose bit 63 is reserved (set by the library). */
+typedef struct gx_table gx_table;
+
+int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t bucket_bytes,
+                    uint64_t hash_seed, gx_table **out);
+void gx_table_free(gx_table *t);
+
+/* Offers n keys (n * key_words words, host memory) to the table.  is_new
+ * (optional, n bytes) receives 1 for keys that were inserted by this call.
+ * *inserted receives the number of new keys.  Returns GX_E_CAPACITY when
+ * the table is full. */
+int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
+                         uint64_t *inserted);
+
+/* Same with keys already resident in device memory (a device pointer);
+ * timed with HIP events into *kernel_ms when non-NULL. */
+int gx_table_insert_device(gx_table *t, const uint64_t *dev_keys, uint64_t n,
+                           uint64_t *inserted, float *kernel_ms);
+
+/* Number of occupied slots (cu:687-704). */
+int gx_table_count(gx_table *t, uint64_t *count);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int gx_device_count(int *count);
+
+/* Device buffer helpers for callers without their own allocator. */
+int gx_device_alloc(uint64_t bytes, void **dev_ptr);
+int gx_device_free(void *dev_ptr);
+int gx_copy_to_device(void *dev_dst, const void *host_src, uint64_t bytes);
+int gx_copy_to_host(void *host_dst, const void *dev_src, uint64_t bytes);
+
+/* Streaming copy of `bytes` bytes device-to-device, timed with HIP events:
+ * the measured peak the hash phase is compared against. */
+int gx_bench_copy(uint64_t bytes, int reps, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,695 @@
+/*
+ * gx_oracle.c -- CPU oracle for the GPUexplore gather/explore path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gx_oracle.h).  Plain C, single thread.
+ * Every decode step follows the reference kernel's arithmetic on the raw gpf
+ * arrays (cu:N = GPUexplore/src/GPUexplore.cu line N); nothing is
+ * precompiled, so this stays independent of the product's model compiler.
+ */
+#include "gx_oracle.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------ */
+/* bit helpers (cu:153-161, 205): (x >> start) & ((1 << len) - 1), with
+ * len == 32 handled instead of being undefined. */
+static inline uint32_t getbits(uint32_t x, uint32_t start, uint32_t len) {
+  if (len == 0 || start >= 32) return 0;
+  uint64_t mask = (len >= 32) ? 0xFFFFFFFFull : ((1ull << len) - 1ull);
+  return (uint32_t)(((uint64_t)x >> start) & mask);
+}
+
+/* ------------------------------------------------------------------ */
+/* gpf loader (cu:1885-1983).  fgets into a 256-byte buffer, then atoi,
+ * which on glibc is (int)strtol: values up to 2^32-1 wrap into uint32. */
+typedef struct {
+  FILE *fp;
+  int eof;
+} rdr;
+
+static int rd(rdr *r, uint32_t *v) {
+  char buf[256];
+  if (!fgets(buf, sizeof buf, r->fp)) {
+    r->eof = 1;
+    return ORC_ERR_FORMAT;
+  }
+  *v = (uint32_t)strtol(buf, NULL, 10);
+  return ORC_OK;
+}
+
+static int rd_arr(rdr *r, uint32_t n, uint32_t **out) {
+  uint32_t *a = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
+  if (!a) return ORC_ERR_NOMEM;
+  for (uint32_t i = 0; i < n; i++) {
+    if (rd(r, &a[i]) != ORC_OK) {
+      free(a);
+      return ORC_ERR_FORMAT;
+    }
+  }
+  *out = a;
+  return ORC_OK;
+}
+
+void orc_free(orc_model *m) {
+  if (!m) return;
+  free(m->firstbit);
+  free(m->bits_state);
+  free(m->proc_offsets_start);
+  free(m->proc_offsets);
+  free(m->proc_trans);
+  free(m->syncbits_offsets);
+  free(m->syncbits);
+  free(m);
+}
+
+int orc_load_gpf(const char *path_noext, int por, orc_model **out) {
+  char fn[4096];
+  snprintf(fn, sizeof fn, "%s.gpf", path_noext); /* cu:1814-1815 */
+  FILE *fp = fopen(fn, "r");
+  if (!fp) return ORC_ERR_NOFILE; /* cu:1984-1986 */
+  orc_model *m = (orc_model *)calloc(1, sizeof *m);
+  rdr r = {fp, 0};
+  int rc = ORC_ERR_FORMAT;
+  uint32_t n_offsets;
+  if (!m) {
+    fclose(fp);
+    return ORC_ERR_NOMEM;
+  }
+  if (rd(&r, &m->liveness)) goto fail;              /* cu:1888 */
+  if (rd(&r, &m->nr_procs)) goto fail;              /* cu:1899 */
+  if (rd(&r, &m->bits_act)) goto fail;              /* cu:1903 */
+  if (rd(&r, &m->proc_nrstates)) goto fail;         /* cu:1907 */
+  if (rd(&r, &m->bits_statevector)) goto fail;      /* cu:1911 */
+  m->bits_statevector += por ? 1u : 0u;             /* cu:1912 */
+  if (m->nr_procs == 0 || m->nr_procs > ORC_MAX_PROCS) {
+    rc = ORC_ERR_LIMIT;
+    goto fail;
+  }
+  if ((rc = rd_arr(&r, m->nr_procs + 1, &m->firstbit))) goto fail; /* cu:1915-1921 */
+  m->sv_nints = (m->bits_statevector + 31) / 32;                   /* cu:1923 */
+  if ((rc = rd_arr(&r, m->nr_procs, &m->bits_state))) goto fail;   /* cu:1924-1930 */
+  rc = ORC_ERR_FORMAT;
+  if (rd(&r, &m->nbits_offset)) goto fail;          /* cu:1931 */
+  if (rd(&r, &m->max_buf_ints)) goto fail;          /* cu:1935 */
+  if ((rc = rd_arr(&r, m->nr_procs + 1, &m->proc_offsets_start))) goto fail; /* cu:1939-1944 */
+  n_offsets = m->proc_offsets_start[m->nr_procs];
+  if ((rc = rd_arr(&r, n_offsets, &m->proc_offsets))) goto fail;  /* cu:1945-1950 */
+  rc = ORC_ERR_FORMAT;
+  if (rd(&r, &m->nr_trans)) goto fail;              /* cu:1951 */
+  if ((rc = rd_arr(&r, m->nr_trans, &m->proc_trans))) goto fail;  /* cu:1955-1960 */
+  rc = ORC_ERR_FORMAT;
+  if (rd(&r, &m->nbits_syncbits_offset)) goto fail; /* cu:1962 */
+  if (rd(&r, &m->nr_syncbits_offsets)) goto fail;   /* cu:1965 */
+  if ((rc = rd_arr(&r, m->nr_syncbits_offsets, &m->syncbits_offsets))) goto fail; /* cu:1968-1973 */
+  rc = ORC_ERR_FORMAT;
+  if (rd(&r, &m->nr_syncbits)) goto fail;           /* cu:1974 */
+  if ((rc = rd_arr(&r, m->nr_syncbits, &m->syncbits))) goto fail; /* cu:1977-1982 */
+  fclose(fp);
+  /* consistency needed for safe decoding */
+  for (uint32_t p = 0; p < m->nr_procs; p++) {
+    if (m->firstbit[p + 1] < m->firstbit[p] || m->firstbit[p + 1] - m->firstbit[p] > 32) {
+      orc_free(m);
+      return ORC_ERR_RANGE;
+    }
+    if (m->bits_state[p] == 0 || m->bits_state[p] > 31) {
+      orc_free(m);
+      return ORC_ERR_RANGE;
+    }
+  }
+  if (m->firstbit[m->nr_procs] > 32u * ORC_MAX_WORDS || m->nbits_offset == 0 ||
+      m->nbits_offset > 32 || m->nbits_syncbits_offset == 0 || m->nbits_syncbits_offset > 32 ||
+      m->bits_act > 30) {
+    orc_free(m);
+    return ORC_ERR_RANGE;
+  }
+  m->words = (m->firstbit[m->nr_procs] + 31) / 32;
+  if (m->words == 0) m->words = 1;
+  *out = m;
+  return ORC_OK;
+fail:
+  fclose(fp);
+  orc_free(m);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* vector codec: GETSTATEVECTORSTATE / SETSTATEVECTORSTATE (cu:162-178).
+ * Field p occupies bits [fb[p], fb[p+1]) of the little-endian word array
+ * and may straddle two words. */
+uint32_t orc_get(const orc_model *m, const uint32_t *v, uint32_t p) {
+  uint32_t lo = m->firstbit[p], hi = m->firstbit[p + 1], w = hi - lo;
+  if (w == 0) return 0;
+  uint64_t x = v[lo / 32];
+  if (lo / 32 != (hi - 1) / 32) x |= (uint64_t)v[lo / 32 + 1] << 32;
+  x >>= (lo % 32);
+  return (uint32_t)(x & ((w >= 32) ? 0xFFFFFFFFull : ((1ull << w) - 1ull)));
+}
+
+void orc_set(const orc_model *m, uint32_t *v, uint32_t p, uint32_t x) {
+  uint32_t lo = m->firstbit[p], hi = m->firstbit[p + 1], w = hi - lo;
+  if (w == 0) return;
+  int straddle = (lo / 32 != (hi - 1) / 32);
+  uint64_t mask = ((w >= 32) ? 0xFFFFFFFFull : ((1ull << w) - 1ull)) << (lo % 32);
+  uint64_t cur = v[lo / 32];
+  if (straddle) cur |= (uint64_t)v[lo / 32 + 1] << 32;
+  cur = (cur & ~mask) | (((uint64_t)x << (lo % 32)) & mask);
+  v[lo / 32] = (uint32_t)cur;
+  if (straddle) v[lo / 32 + 1] = (uint32_t)(cur >> 32);
+}
+
+/* ------------------------------------------------------------------ */
+/* transition offsets of process p in local state s (cu:883-897) */
+static int proc_range(const orc_model *m, uint32_t p, uint32_t s, uint32_t *o1, uint32_t *o2) {
+  uint32_t nb = m->nbits_offset, k = 32 / nb;
+  uint32_t index = s / k, pos = s - index * k;
+  uint64_t at = (uint64_t)m->proc_offsets_start[p] + index;
+  uint32_t n_off = m->proc_offsets_start[m->nr_procs];
+  if (at >= n_off) return ORC_ERR_RANGE;
+  uint32_t tmp = m->proc_offsets[at];
+  *o1 = getbits(tmp, pos * nb, nb);
+  if (pos == k - 1) {
+    if (at + 1 >= n_off) return ORC_ERR_RANGE;
+    *o2 = getbits(m->proc_offsets[at + 1], 0, nb);
+  } else {
+    *o2 = getbits(tmp, (pos + 1) * nb, nb);
+  }
+  if (*o1 > *o2 || *o2 > m->nr_trans) return ORC_ERR_RANGE;
+  return ORC_OK;
+}
+
+/* transition entry fields (cu:143, 157-159) */
+static inline uint32_t entry_sync(uint32_t t) { return t & 1u; }
+static inline uint32_t entry_label(const orc_model *m, uint32_t t) { return getbits(t, 1, m->bits_act); }
+static inline uint32_t entry_capacity(const orc_model *m, uint32_t p) {
+  return (31u - m->bits_act) / m->bits_state[p];
+}
+/* target i of entry t for process p, as stored: state id + 1, 0 = end */
+static inline uint32_t entry_target(const orc_model *m, uint32_t t, uint32_t i, uint32_t p) {
+  return getbits(t, 1 + m->bits_act + i * m->bits_state[p], m->bits_state[p]);
+}
+
+/* rules of label a: syncbits ints [so1, so2) (cu:999-1008, 1466-1476);
+ * N-bit rule fields, floor(32/N) per int (cu:205, 1480-1487) */
+static int label_rules(const orc_model *m, uint32_t a, uint32_t *so1, uint32_t *so2) {
+  uint32_t nb = m->nbits_syncbits_offset, k = 32 / nb;
+  uint32_t i = a / k, pos = a - i * k;
+  if (i >= m->nr_syncbits_offsets) return ORC_ERR_RANGE;
+  uint32_t l = m->syncbits_offsets[i];
+  *so1 = getbits(l, pos * nb, nb);
+  if (pos == k - 1) {
+    if (i + 1 >= m->nr_syncbits_offsets) return ORC_ERR_RANGE;
+    *so2 = getbits(m->syncbits_offsets[i + 1], 0, nb);
+  } else {
+    *so2 = getbits(l, (pos + 1) * nb, nb);
+  }
+  if (*so1 > *so2 || *so2 > m->nr_syncbits) return ORC_ERR_RANGE;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* per-state decoded view */
+#define MAXT 4096
+typedef struct {
+  uint32_t n;
+  uint32_t sp[ORC_MAX_PROCS], o1[ORC_MAX_PROCS], o2[ORC_MAX_PROCS];
+  uint32_t nlabels;
+  uint32_t labels[MAXT]; /* distinct sync labels present, ascending */
+} sview;
+
+static int cmp_u32(const void *a, const void *b) {
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return (x > y) - (x < y);
+}
+
+static int view_state(const orc_model *m, const uint32_t *s, sview *v) {
+  v->n = m->nr_procs;
+  v->nlabels = 0;
+  for (uint32_t p = 0; p < v->n; p++) {
+    v->sp[p] = orc_get(m, s, p);
+    int rc = proc_range(m, p, v->sp[p], &v->o1[p], &v->o2[p]);
+    if (rc) return rc;
+    for (uint32_t e = v->o1[p]; e < v->o2[p]; e++) {
+      uint32_t t = m->proc_trans[e];
+      if (entry_sync(t)) {
+        if (v->nlabels >= MAXT) return ORC_ERR_LIMIT;
+        v->labels[v->nlabels++] = entry_label(m, t);
+      }
+    }
+  }
+  qsort(v->labels, v->nlabels, sizeof(uint32_t), cmp_u32);
+  uint32_t u = 0;
+  for (uint32_t i = 0; i < v->nlabels; i++)
+    if (u == 0 || v->labels[u - 1] != v->labels[i]) v->labels[u++] = v->labels[i];
+  v->nlabels = u;
+  return ORC_OK;
+}
+
+/* T_p(a): concatenated targets (value id+1 decoded to id) of p's a-entries */
+static uint32_t sync_targets(const orc_model *m, const sview *v, uint32_t p, uint32_t a,
+                             uint32_t *out, uint32_t cap) {
+  uint32_t n = 0, c = entry_capacity(m, p);
+  for (uint32_t e = v->o1[p]; e < v->o2[p]; e++) {
+    uint32_t t = m->proc_trans[e];
+    if (!entry_sync(t) || entry_label(m, t) != a) continue;
+    for (uint32_t i = 0; i < c; i++) {
+      uint32_t x = entry_target(m, t, i, p);
+      if (x == 0) break;
+      if (n < cap) out[n] = x - 1;
+      n++;
+    }
+  }
+  return n;
+}
+
+static int has_label(const orc_model *m, const sview *v, uint32_t p, uint32_t a) {
+  for (uint32_t e = v->o1[p]; e < v->o2[p]; e++) {
+    uint32_t t = m->proc_trans[e];
+    if (entry_sync(t) && entry_label(m, t) == a) return 1;
+  }
+  return 0;
+}
+
+/* successor visitor: kind 0 = local move of process `who`, kind 1 = rule `rule` */
+typedef int (*succ_fn)(void *ctx, const uint32_t *t, int kind, uint32_t who, uint64_t rule);
+
+typedef struct {
+  int any_local_entry;    /* cu:910-913: a local entry exists */
+  int any_enabled_rule;   /* cu:1033-1035 */
+  uint64_t cluster0[ORC_MAX_PROCS];   /* cu:1347, 1616-1624 */
+  uint64_t local_succ;    /* processes with >= 1 local successor */
+  uint64_t in_enabled;    /* processes in some enabled rule with >= 1 successor */
+} sinfo;
+
+/* Enumerate successors of s.  Locals of processes in `allow`, then for every
+ * sync label in ascending order every enabled rule r with r subset of `allow`
+ * (cu:904-949, 951-1130; POR ample filter cu:1409, 1541).  Fills `info` from
+ * the full (unfiltered) enabledness. */
+static int enumerate(const orc_model *m, const uint32_t *s, const sview *v, uint64_t allow,
+                     succ_fn fn, void *ctx, sinfo *info) {
+  uint32_t W = m->words, N = m->nr_procs;
+  uint32_t t[ORC_MAX_WORDS];
+  static uint32_t tg[ORC_MAX_PROCS][MAXT];
+  uint32_t tn[ORC_MAX_PROCS];
+  memset(info, 0, sizeof *info);
+  for (uint32_t p = 0; p < N; p++) info->cluster0[p] = 1ull << p;
+  /* local successors */
+  for (uint32_t p = 0; p < N; p++) {
+    uint32_t c = entry_capacity(m, p);
+    for (uint32_t e = v->o1[p]; e < v->o2[p]; e++) {
+      uint32_t ent = m->proc_trans[e];
+      if (entry_sync(ent)) continue;
+      info->any_local_entry = 1;
+      for (uint32_t i = 0; i < c; i++) {
+        uint32_t x = entry_target(m, ent, i, p);
+        if (x == 0) break;
+        info->local_succ |= 1ull << p;
+        if (!((allow >> p) & 1)) continue;
+        memcpy(t, s, W * sizeof(uint32_t));
+        orc_set(m, t, p, x - 1);
+        if (fn) {
+          int rc = fn(ctx, t, 0, p, 0);
+          if (rc) return rc;
+        }
+      }
+    }
+  }
+  /* synchronising successors, labels ascending */
+  for (uint32_t li = 0; li < v->nlabels; li++) {
+    uint32_t a = v->labels[li];
+    uint64_t en = 0;
+    for (uint32_t p = 0; p < N; p++)
+      if (has_label(m, v, p, a)) en |= 1ull << p;
+    uint32_t so1, so2;
+    int rc = label_rules(m, a, &so1, &so2);
+    if (rc) return rc;
+    uint32_t per = 32 / N;
+    for (uint32_t j = so1; j < so2; j++) {
+      for (uint32_t f = 0; f < per; f++) {
+        uint64_t r = getbits(m->syncbits[j], f * N, N);
+        if (r == 0) continue;
+        for (uint32_t p = 0; p < N; p++)
+          if (((r >> p) & 1) && ((en >> p) & 1)) info->cluster0[p] |= r;
+        if (r & ~en) continue; /* not enabled */
+        info->any_enabled_rule = 1;
+        /* product of target lists, lowest process varies fastest (cu:1084-1113) */
+        uint32_t mem[ORC_MAX_PROCS], nm = 0;
+        uint64_t prod = 1;
+        for (uint32_t p = 0; p < N; p++) {
+          if (!((r >> p) & 1)) continue;
+          tn[p] = sync_targets(m, v, p, a, tg[p], MAXT);
+          if (tn[p] > MAXT) return ORC_ERR_LIMIT;
+          mem[nm++] = p;
+          prod *= tn[p];
+        }
+        if (prod == 0) continue;
+        info->in_enabled |= r;
+        if (r & ~allow) continue;
+        uint32_t dig[ORC_MAX_PROCS] = {0};
+        memcpy(t, s, W * sizeof(uint32_t));
+        for (uint32_t i = 0; i < nm; i++) orc_set(m, t, mem[i], tg[mem[i]][0]);
+        for (;;) {
+          if (fn) {
+            rc = fn(ctx, t, 1, 0, r);
+            if (rc) return rc;
+          }
+          uint32_t i = 0;
+          for (; i < nm; i++) {
+            uint32_t p = mem[i];
+            if (++dig[i] < tn[p]) {
+              orc_set(m, t, p, tg[p][dig[i]]);
+              break;
+            }
+            dig[i] = 0;
+            orc_set(m, t, p, tg[p][0]);
+          }
+          if (i == nm) break;
+        }
+      }
+    }
+  }
+  return ORC_OK;
+}
+
+typedef struct {
+  uint32_t *out;
+  size_t cap, n;
+  uint32_t W;
+} collect_ctx;
+
+static int collect(void *c, const uint32_t *t, int kind, uint32_t who, uint64_t rule) {
+  (void)kind;
+  (void)who;
+  (void)rule;
+  collect_ctx *cc = (collect_ctx *)c;
+  if (cc->n < cc->cap) memcpy(cc->out + cc->n * cc->W, t, cc->W * sizeof(uint32_t));
+  cc->n++;
+  return 0;
+}
+
+long orc_successors(const orc_model *m, const uint32_t *state, uint32_t *out, size_t cap,
+                    int *deadlock) {
+  sview v;
+  sinfo info;
+  int rc = view_state(m, state, &v);
+  if (rc) return rc;
+  collect_ctx cc = {out, cap, 0, m->words};
+  rc = enumerate(m, state, &v, ~0ull, collect, &cc, &info);
+  if (rc) return rc;
+  if (deadlock) *deadlock = !(info.any_local_entry || info.any_enabled_rule); /* cu:1134-1151 */
+  return (long)cc.n;
+}
+
+/* ------------------------------------------------------------------ */
+/* cample cluster selection (cu:1633-1672) from cluster0 and prov */
+static uint64_t select_cluster(uint32_t N, const uint64_t *cluster0, uint64_t prov) {
+  uint64_t c[ORC_MAX_PROCS], fin[ORC_MAX_PROCS];
+  uint64_t all = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
+  for (uint32_t p = 0; p < N; p++)
+    c[p] = ((prov >> p) & 1) ? cluster0[p] : (cluster0[p] & ~(1ull << p)); /* cu:1635-1637 */
+  for (uint32_t p = 0; p < N; p++) {
+    uint64_t K = c[p], done = 0;
+    int ok = 0;
+    /* closure: K is closed under i in K => c(i) subset of K (cu:1641-1649) */
+    while (K & ~done) {
+      uint32_t i = (uint32_t)__builtin_ctzll(K & ~done);
+      done |= 1ull << i;
+      if ((c[i] >> i) & 1) ok = 1;
+      K |= c[i];
+    }
+    fin[p] = ok ? K : 0; /* cu:1651-1655 */
+  }
+  int best = (int)N;
+  uint64_t C = all; /* cu:1658-1659 */
+  for (uint32_t p = 0; p < N; p++) {
+    int pc = __builtin_popcountll(fin[p]);
+    if (fin[p] && pc < best) { /* first minimal, strictly smaller than N (cu:1661) */
+      best = pc;
+      C = fin[p];
+    }
+  }
+  return C;
+}
+
+uint64_t orc_por_cluster(const orc_model *m, const uint32_t *state) {
+  sview v;
+  sinfo info;
+  if (view_state(m, state, &v)) return 0;
+  if (enumerate(m, state, &v, 0, NULL, NULL, &info)) return 0;
+  return select_cluster(m->nr_procs, info.cluster0, info.local_succ | info.in_enabled);
+}
+
+/* ------------------------------------------------------------------ */
+/* visited set: open addressing, key = W words, value = level + 1 (0 = empty) */
+typedef struct {
+  uint32_t W;
+  uint64_t cap, n, mask, seed;
+  uint32_t *keys;
+  uint32_t *lvl;
+} vset;
+
+static inline uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+static inline uint64_t vhash(const vset *s, const uint32_t *k) {
+  uint64_t h = s->seed ^ 0x9e3779b97f4a7c15ull;
+  for (uint32_t i = 0; i < s->W; i++) h = mix64(h ^ k[i]) + i;
+  return h;
+}
+
+static int vset_init(vset *s, uint32_t W, uint64_t cap, uint64_t seed) {
+  s->W = W;
+  s->cap = cap;
+  s->mask = cap - 1;
+  s->n = 0;
+  s->seed = seed;
+  s->keys = (uint32_t *)malloc(cap * W * sizeof(uint32_t));
+  s->lvl = (uint32_t *)calloc(cap, sizeof(uint32_t));
+  return (s->keys && s->lvl) ? ORC_OK : ORC_ERR_NOMEM;
+}
+
+static void vset_free(vset *s) {
+  free(s->keys);
+  free(s->lvl);
+}
+
+/* returns stored level+1 if present, 0 if absent; *slot gets the slot */
+static uint32_t vset_probe(const vset *s, const uint32_t *k, uint64_t *slot) {
+  uint64_t i = vhash(s, k) & s->mask;
+  for (;;) {
+    uint32_t l = s->lvl[i];
+    if (l == 0) {
+      *slot = i;
+      return 0;
+    }
+    if (memcmp(s->keys + i * s->W, k, s->W * sizeof(uint32_t)) == 0) {
+      *slot = i;
+      return l;
+    }
+    i = (i + 1) & s->mask;
+  }
+}
+
+static int vset_grow(vset *s) {
+  vset t;
+  int rc = vset_init(&t, s->W, s->cap * 2, s->seed);
+  if (rc) return rc;
+  for (uint64_t i = 0; i < s->cap; i++) {
+    if (!s->lvl[i]) continue;
+    uint64_t sl;
+    vset_probe(&t, s->keys + i * s->W, &sl);
+    memcpy(t.keys + sl * s->W, s->keys + i * s->W, s->W * sizeof(uint32_t));
+    t.lvl[sl] = s->lvl[i];
+    t.n++;
+  }
+  vset_free(s);
+  *s = t;
+  return ORC_OK;
+}
+
+/* insert with level; returns 1 if new, 0 if present, <0 on error */
+static int vset_put(vset *s, const uint32_t *k, uint32_t level) {
+  if ((s->n + 1) * 2 > s->cap) {
+    int rc = vset_grow(s);
+    if (rc) return rc;
+  }
+  uint64_t sl;
+  if (vset_probe(s, k, &sl)) return 0;
+  memcpy(s->keys + sl * s->W, k, s->W * sizeof(uint32_t));
+  s->lvl[sl] = level + 1;
+  s->n++;
+  return 1;
+}
+
+/* ------------------------------------------------------------------ */
+typedef struct {
+  uint32_t *v;
+  uint64_t n, cap;
+  uint32_t W;
+} vlist;
+
+static int vlist_push(vlist *l, const uint32_t *k) {
+  if (l->n == l->cap) {
+    uint64_t nc = l->cap ? l->cap * 2 : 1024;
+    uint32_t *nv = (uint32_t *)realloc(l->v, nc * l->W * sizeof(uint32_t));
+    if (!nv) return ORC_ERR_NOMEM;
+    l->v = nv;
+    l->cap = nc;
+  }
+  memcpy(l->v + l->n * l->W, k, l->W * sizeof(uint32_t));
+  l->n++;
+  return ORC_OK;
+}
+
+typedef struct {
+  const orc_model *m;
+  const orc_options *o;
+  vset *set;
+  vlist *next;
+  uint32_t level;      /* level being expanded */
+  uint64_t transitions, full;
+  int violation;
+  int check_safety;    /* safety checked in on_succ (full expansion only) */
+  /* proviso bookkeeping */
+  uint64_t local_new;  /* processes with a new local successor */
+  uint64_t rules_new[MAXT];
+  uint64_t nrules_new;
+} bfs_ctx;
+
+static int is_violating(const orc_model *m, const uint32_t *t) {
+  return orc_get(m, t, m->nr_procs - 1) == 1; /* cu:926-935, 1063-1069 */
+}
+
+/* ample / full successor: count it and insert it */
+static int on_succ(void *c, const uint32_t *t, int kind, uint32_t who, uint64_t rule) {
+  (void)kind;
+  (void)who;
+  (void)rule;
+  bfs_ctx *b = (bfs_ctx *)c;
+  b->transitions++;
+  if (b->check_safety && is_violating(b->m, t)) b->violation = 1;
+  int rc = vset_put(b->set, t, b->level + 1);
+  if (rc < 0) return rc;
+  if (rc == 1) return vlist_push(b->next, t);
+  return ORC_OK;
+}
+
+/* full expansion visitor under POR: safety check on every successor
+ * (cu:1384-1393, 1519-1525) */
+static int on_full(void *c, const uint32_t *t, int kind, uint32_t who, uint64_t rule) {
+  (void)kind;
+  (void)who;
+  (void)rule;
+  bfs_ctx *b = (bfs_ctx *)c;
+  b->full++;
+  if (b->o->property == ORC_PROP_SAFETY && is_violating(b->m, t)) b->violation = 1;
+  return ORC_OK;
+}
+
+/* cycle proviso visitor: a successor is new unless visited at level <= L */
+static int on_proviso(void *c, const uint32_t *t, int kind, uint32_t who, uint64_t rule) {
+  bfs_ctx *b = (bfs_ctx *)c;
+  uint64_t sl;
+  uint32_t l = vset_probe(b->set, t, &sl);
+  int isnew = !(l != 0 && (l - 1) <= b->level);
+  if (!isnew) return ORC_OK;
+  if (kind == 0) {
+    b->local_new |= 1ull << who;
+  } else {
+    for (uint64_t i = 0; i < b->nrules_new; i++)
+      if (b->rules_new[i] == rule) return ORC_OK;
+    if (b->nrules_new < MAXT) b->rules_new[b->nrules_new++] = rule;
+  }
+  return ORC_OK;
+}
+
+int orc_explore(const orc_model *m, const orc_options *o, orc_result *res) {
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  memset(res, 0, sizeof *res);
+  uint32_t W = m->words;
+  vset set;
+  vlist cur = {NULL, 0, 0, W}, nxt = {NULL, 0, 0, W};
+  int rc = vset_init(&set, W, 1u << 16, o->hash_seed);
+  if (rc) return rc;
+  uint32_t s0[ORC_MAX_WORDS] = {0}; /* all processes in state 0 (cu:673-676) */
+  vset_put(&set, s0, 0);
+  vlist_push(&cur, s0);
+  bfs_ctx b;
+  memset(&b, 0, sizeof b);
+  b.m = m;
+  b.o = o;
+  b.set = &set;
+  b.next = &nxt;
+  uint32_t level = 0;
+  while (cur.n > 0) {
+    b.level = level;
+    for (uint64_t i = 0; i < cur.n; i++) {
+      const uint32_t *s = cur.v + i * W;
+      sview v;
+      sinfo info;
+      rc = view_state(m, s, &v);
+      if (rc) goto out;
+      if (!o->por) {
+        /* safety on every generated successor (cu:926-935, 1063-1069) */
+        b.check_safety = (o->property == ORC_PROP_SAFETY);
+        rc = enumerate(m, s, &v, ~0ull, on_succ, &b, &info);
+        if (rc) goto out;
+      } else {
+        /* full expansion pass: enabledness, clusters, safety */
+        rc = enumerate(m, s, &v, ~0ull, on_full, &b, &info);
+        if (rc) goto out;
+        uint64_t prov;
+        if (!o->cycle_proviso) {
+          prov = info.local_succ | info.in_enabled; /* cu:1395-1398, 1527-1530 */
+        } else {
+          sinfo tmp;
+          b.local_new = 0;
+          b.nrules_new = 0;
+          rc = enumerate(m, s, &v, ~0ull, on_proviso, &b, &tmp);
+          if (rc) goto out;
+          prov = b.local_new;
+          for (uint64_t r = 0; r < b.nrules_new; r++) prov |= b.rules_new[r];
+        }
+        uint64_t C = select_cluster(m->nr_procs, info.cluster0, prov);
+        sinfo info2;
+        rc = enumerate(m, s, &v, C, on_succ, &b, &info2);
+        if (rc) goto out;
+      }
+      if (o->property == ORC_PROP_DEADLOCK && !(info.any_local_entry || info.any_enabled_rule))
+        b.violation = 1; /* cu:1134-1151, 1679-1696 */
+    }
+    level++;
+    res->levels = level;
+    /* swap */
+    vlist t = cur;
+    cur = nxt;
+    nxt = t;
+    nxt.n = 0;
+    if (b.violation) break; /* level-granular early stop */
+    if (o->max_states && set.n >= o->max_states) {
+      res->truncated = 1;
+      break;
+    }
+  }
+  rc = ORC_OK;
+out:
+  res->states = set.n;
+  res->transitions = b.transitions;
+  res->full_successors = o->por ? b.full : b.transitions;
+  res->violation = b.violation;
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  res->seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  vset_free(&set);
+  free(cur.v);
+  free(nxt.v);
+  return rc;
+}

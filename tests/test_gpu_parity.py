@@ -1,0 +1,157 @@
+"""HIP path against the oracle: identical state, transition and level counts
+and identical verdicts (bit-exact: integer work), plus visited-set parity
+against a Python set and invariance to launch geometry, hash seed and
+bucket width."""
+
+import json
+import os
+import random
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from gpuexplore_amd import gen
+from test_oracle import small_models
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("states", "transitions", "levels", "violation")
+MODES = [(False, False), (True, False), (True, True)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_run(gx, path, prop=0, por=False, proviso=False, **kw):
+    m = gx.Model.load(path, por=por)
+    kw.setdefault("q_ints", 1 << 22)
+    r = gx.explore(m, property=prop, por=int(por), cycle_proviso=int(proviso), **kw)
+    r["violation"] = bool(r["violation"])
+    return r
+
+
+def same(a, b):
+    return all(a[k] == b[k] for k in KEYS)
+
+
+def test_tiny2_golden(gpu):
+    exp = json.load(open(os.path.join(GOLDEN, "tiny2.json")))["explore"]
+    p = os.path.join(GOLDEN, "tiny2")
+    for name, prop in [("none", 0), ("deadlock", 1), ("safety", 2)]:
+        say("tiny2", name)
+        r = gpu_run(gpu, p, prop)
+        assert all(r[k] == exp[name][k] for k in KEYS), (name, r)
+    r = gpu_run(gpu, p, 1, por=True)
+    assert all(r[k] == exp["por"][k] for k in KEYS), r
+
+
+def test_closed_forms(gpu, model_dir):
+    cases = [
+        (gen.independent_cycles([3, 4, 5]), 60, 180, False),
+        (gen.independent_chains(4, 3), 256, 768, True),
+        (gen.barrier_flip(6), 64, 6 * 32 + 1, False),
+    ]
+    for i, (net, st, tr, dead) in enumerate(cases):
+        p = os.path.join(model_dir, f"cf{i}")
+        net.write(p)
+        say("closed form", i)
+        r = gpu_run(gpu, p, 1)
+        assert (r["states"], r["transitions"], r["violation"]) == (st, tr, dead)
+    p = os.path.join(model_dir, "chainpor")
+    gen.independent_chains(5, 2).write(p)
+    r = gpu_run(gpu, p, 1, por=True)
+    assert (r["states"], r["transitions"], r["violation"]) == (11, 10, True)
+
+
+@pytest.mark.parametrize("name,net", small_models(), ids=[n for n, _ in small_models()])
+def test_small_models_all_modes(gpu, oracle_mod, model_dir, name, net):
+    p = os.path.join(model_dir, name)
+    net.write(p)
+    for prop in (0, 1, 2):
+        for por, prov in MODES:
+            say(name, prop, por, prov)
+            a = oracle_mod.explore(p, prop, por, prov)
+            b = gpu_run(gpu, p, prop, por, prov)
+            assert same(a, b), (prop, por, prov, a, b)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_config1(gpu, oracle_mod, model_dir, seed):
+    p = os.path.join(model_dir, "c1")
+    gen.config1(seed).write(p)
+    for prop, por, prov in [(1, False, False), (0, True, False), (2, True, True)]:
+        say("config1", seed, prop, por, prov)
+        a = oracle_mod.explore(p, prop, por, prov)
+        b = gpu_run(gpu, p, prop, por, prov)
+        assert same(a, b), (a, b)
+
+
+def test_invariance_launch_seed_bucket(gpu, oracle_mod, model_dir):
+    p = os.path.join(model_dir, "c1")
+    gen.config1(2).write(p)
+    a = oracle_mod.explore(p, 1)
+    for kw in [dict(blocks=1, threads=64), dict(blocks=3, threads=128, kernel_iters=4),
+               dict(threads=512, kernel_iters=16), dict(hash_seed=12345),
+               dict(bucket_bytes=64), dict(bucket_bytes=256), dict(cand_capacity=4096)]:
+        say("invariance", kw)
+        b = gpu_run(gpu, p, 1, **kw)
+        assert same(a, b), (kw, a, b)
+
+
+def test_retry_path_por_proviso(gpu, oracle_mod, model_dir):
+    """a tiny successor buffer forces deferred states; counts are unchanged"""
+    p = os.path.join(model_dir, "c1")
+    gen.config1(4).write(p)
+    a = oracle_mod.explore(p, 2, True, True)
+    say("retry proviso")
+    b = gpu_run(gpu, p, 2, True, True, cand_capacity=2048)
+    assert same(a, b), (a, b)
+
+
+@pytest.mark.parametrize("W", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("bb", [64, 128, 256])
+def test_table_vs_set(gpu, W, bb):
+    if bb // 8 < W:
+        pytest.skip("slot wider than bucket")
+    rng = np.random.default_rng(W * 1000 + bb)
+    n = 20000
+    base = rng.integers(0, 1 << 62, size=(n // 4, W), dtype=np.uint64)
+    keys = base[rng.integers(0, len(base), size=n)]  # ~4x duplication
+    say("table", W, bb)
+    t = gpu.Table(1 << 20, W, bb, seed=W)
+    flags, ins, rc = t.insert(keys.ravel())
+    assert rc == 0
+    uniq = {tuple(r) for r in base}
+    assert ins == len({tuple(r) for r in keys})
+    # exactly one copy of every distinct key was new
+    seen = {}
+    for i, row in enumerate(map(tuple, keys)):
+        seen.setdefault(row, 0)
+        seen[row] += int(flags[i])
+    assert all(v == 1 for v in seen.values())
+    # a second pass finds everything
+    flags2, ins2, rc = t.insert(keys.ravel())
+    assert ins2 == 0 and not flags2.any()
+    assert t.count() == ins
+    assert len(uniq) >= ins
+
+
+def test_table_full_reports(gpu):
+    t = gpu.Table(256, 1, 128)  # 8 buckets x 16 slots
+    keys = np.arange(1, 1001, dtype=np.uint64)
+    say("table full")
+    _, ins, rc = t.insert(keys)
+    assert rc == -10
+    assert ins <= 128
+
+
+def test_tiny_table_explore_reports_full(gpu, model_dir):
+    p = os.path.join(model_dir, "c1")
+    gen.config1(2).write(p)
+    say("tiny table explore")
+    r = gpu_run(gpu, p, 1, q_ints=1024)
+    assert r["table_full"] == 1

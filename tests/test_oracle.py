@@ -1,0 +1,159 @@
+"""Pins the CPU oracle: hand-computed fixture, closed forms, codec brute
+force, and an independent pure-Python enumerator on small models."""
+
+import json
+import os
+import random
+
+import pytest
+
+from conftest import GOLDEN
+from gpuexplore_amd import gen
+from naive_enum import NaiveModel, pack, unpack
+
+PROPS = {"": 0, "d": 1, "p": 2}
+
+
+def write(net, d, name):
+    path = os.path.join(d, name)
+    net.write(path)
+    return path
+
+
+def test_tiny2_parse(oracle_mod):
+    exp = json.load(open(os.path.join(GOLDEN, "tiny2.json")))
+    m = oracle_mod.OracleModel(os.path.join(GOLDEN, "tiny2"))
+    arr = m.arrays()
+    for k, v in exp["parse"].items():
+        assert arr[k] == v, k
+
+
+def test_tiny2_successors(oracle_mod):
+    exp = json.load(open(os.path.join(GOLDEN, "tiny2.json")))
+    m = oracle_mod.OracleModel(os.path.join(GOLDEN, "tiny2"))
+    for s, succ in exp["successors"].items():
+        got, dead = m.successors([int(s)])
+        assert sorted(w[0] for w in got) == sorted(succ)
+        assert dead == exp["deadlock"][s]
+
+
+def test_tiny2_explore(oracle_mod):
+    exp = json.load(open(os.path.join(GOLDEN, "tiny2.json")))["explore"]
+    p = os.path.join(GOLDEN, "tiny2")
+    for name, prop in [("none", 0), ("deadlock", 1), ("safety", 2)]:
+        r = oracle_mod.explore(p, prop)
+        for k in ("states", "transitions", "levels", "violation"):
+            assert r[k] == exp[name][k], (name, k)
+    r = oracle_mod.explore(p, 1, por=True)
+    for k in ("states", "transitions", "levels", "violation"):
+        assert r[k] == exp["por"][k], k
+
+
+def test_tiny2_por_clusters(oracle_mod):
+    exp = json.load(open(os.path.join(GOLDEN, "tiny2.json")))["por_cluster"]
+    m = oracle_mod.OracleModel(os.path.join(GOLDEN, "tiny2"), por=True)
+    for s, c in exp.items():
+        assert m.por_cluster([int(s)]) == c
+
+
+@pytest.mark.parametrize("lengths", [[3], [2, 3], [3, 4, 5], [2, 2, 2, 2, 2]])
+def test_independent_cycles(oracle_mod, model_dir, lengths):
+    p = write(gen.independent_cycles(lengths), model_dir, "cyc")
+    r = oracle_mod.explore(p, 1)
+    prod = 1
+    for n in lengths:
+        prod *= n
+    assert r["states"] == prod
+    assert r["transitions"] == len(lengths) * prod
+    assert not r["violation"]
+
+
+@pytest.mark.parametrize("n,k", [(1, 4), (3, 2), (4, 3), (5, 1)])
+def test_independent_chains(oracle_mod, model_dir, n, k):
+    p = write(gen.independent_chains(n, k), model_dir, "chain")
+    r = oracle_mod.explore(p, 1)
+    assert r["states"] == (k + 1) ** n
+    assert r["transitions"] == n * k * (k + 1) ** (n - 1)
+    assert r["violation"]
+    assert r["levels"] == n * k + 1
+    # cample POR without proviso: the lowest movable process is selected
+    r = oracle_mod.explore(p, 1, por=True)
+    assert r["states"] == n * k + 1
+    assert r["transitions"] == n * k
+    assert r["violation"]
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8])
+def test_barrier(oracle_mod, model_dir, n):
+    p = write(gen.barrier_flip(n), model_dir, "bar")
+    r = oracle_mod.explore(p, 1)
+    assert r["states"] == 2 ** n
+    assert r["transitions"] == n * 2 ** (n - 1) + 1
+    assert not r["violation"]
+
+
+def test_codec_bruteforce(oracle_mod, model_dir):
+    """orc_get/orc_set against a bit-by-bit reference on layouts with fields
+    straddling word boundaries."""
+    rng = random.Random(7)
+    for trial in range(6):
+        widths = [rng.randint(1, 13) for _ in range(rng.randint(2, 12))]
+        net = gen.independent_cycles([2] * len(widths))
+        net.field_widths = widths
+        p = write(net, model_dir, f"codec{trial}")
+        om = oracle_mod.OracleModel(p)
+        nm = NaiveModel(open(p + ".gpf").read())
+        words = om.words
+        for _ in range(50):
+            s = tuple(rng.randrange(1 << w) for w in widths)
+            v = pack(nm, s, words)
+            for q in range(len(widths)):
+                assert om.get(v, q) == s[q]
+                x = rng.randrange(1 << widths[q])
+                v2 = om.set(v, q, x)
+                s2 = s[:q] + (x,) + s[q + 1:]
+                assert unpack(nm, v2) == s2
+                assert v2 == pack(nm, s2, words)
+
+
+def small_models():
+    out = []
+    for seed in range(1, 4):
+        out.append((f"ring{seed}", gen.config1(seed)))
+        out.append((f"thr{seed}", gen.throttled(4, 8, 2, 2, (0.6, 0.3, 0.1), 2, seed)))
+        out.append((f"mon{seed}", gen.throttled(4, 6, 2, 2, (0.5, 0.4, 0.1), 2, seed, monitor=(0, 1, 2, 3))))
+        out.append((f"tr3{seed}", gen.throttled(3, 12, 3, 3, (0.7, 0.2, 0.1), 3, seed)))
+        wide = gen.throttled(3, 6, 2, 2, (0.5, 0.4, 0.1), 2, seed)
+        wide.field_widths = [17, 16, 31]  # straddles both word boundaries
+        out.append((f"wide{seed}", wide))
+    return out
+
+
+@pytest.mark.parametrize("name,net", small_models(), ids=[n for n, _ in small_models()])
+def test_oracle_vs_naive(oracle_mod, model_dir, name, net):
+    p = write(net, model_dir, name)
+    nm = NaiveModel(open(p + ".gpf").read())
+    for prop in ("", "d", "p"):
+        for por, prov in ((False, False), (True, False), (True, True)):
+            a = nm.explore(prop, por, prov)
+            b = oracle_mod.explore(p, PROPS[prop], por, prov)
+            for k in ("states", "transitions", "levels", "violation"):
+                assert a[k] == b[k], (prop, por, prov, k, a, b)
+
+
+def test_por_subset_and_verdict(oracle_mod, model_dir):
+    for seed in range(1, 6):
+        p = write(gen.config1(seed), model_dir, f"c1_{seed}")
+        full = oracle_mod.explore(p, 1)
+        red = oracle_mod.explore(p, 1, por=True)
+        prov = oracle_mod.explore(p, 1, por=True, proviso=True)
+        assert red["states"] <= full["states"]
+        assert prov["states"] <= full["states"]
+        assert red["violation"] == full["violation"]
+
+
+def test_seed_invariance(oracle_mod, model_dir):
+    p = write(gen.config1(3), model_dir, "c1")
+    a = oracle_mod.explore(p, 1, seed=1)
+    b = oracle_mod.explore(p, 1, seed=987654321)
+    assert (a["states"], a["transitions"], a["levels"]) == (b["states"], b["transitions"], b["levels"])
