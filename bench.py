@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark of the gather/explore path on one MI355X.
+
+Workload (default): the large plain-method model -- a ring of 20 seeded
+random LTSs with 16 local states each (80-bit state vectors), out-degree 3,
+4 local labels, a neighbour rendezvous and a 20-way barrier, token-throttled
+(gpuexplore_amd.gen.config2, seed 1) -- explored without a property in a
+visited set of 2^34 32-bit ints (64 GiB).  One step is one full exploration
+from the initial state (the visited set is reset at the start of the step;
+device buffers are allocated once, before the timed steps).
+
+Prints one JSON line: new states/s (whole job), the FIND-OR-PUT kernel's
+achieved algorithmic bandwidth against the HBM peak, and a CPU baseline (the
+oracle's BFS on a bounded sample of the same workload, one host thread).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E, published
+BYTES_PER_FINDORPUT = 128  # one 128-byte bucket line per lookup, one per insert
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--q-log2", type=int, default=34, help="visited set size: 2^q 32-bit ints")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample-states", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        return 2
+
+    import gpuexplore_amd as gx
+    from gpuexplore_amd import gen
+
+    if gx.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    tmp = tempfile.mkdtemp(prefix="gx_bench_")
+    path = os.path.join(tmp, "config2")
+    gen.config2(args.seed).write(path)
+    model = gx.Model.load(path)
+    info = model.info()
+
+    import ctypes
+    peak = ctypes.c_double(0)
+    gx.lib().gx_bench_copy(4 << 30, 20, ctypes.byref(peak))
+
+    ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE)
+    for i in range(args.warmup):
+        r = ex.run()
+        print(f"warmup {i}: states={r['states']} t_explore={r['t_explore_s']:.3f}s", file=sys.stderr, flush=True)
+    runs = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = ex.run()
+        runs.append(r)
+        print(f"step {i}: states={r['states']} t_explore={r['t_explore_s']:.3f}s "
+              f"t_hash={r['t_hash_s']:.3f}s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    states = runs[0]["states"]
+    if any(r["states"] != states or r["transitions"] != runs[0]["transitions"] for r in runs):
+        print("state counts differ between steps", file=sys.stderr)
+        return 1
+    t_exp = sum(r["t_explore_s"] for r in runs)
+    t_hash = sum(r["t_hash_s"] for r in runs)
+    lookups = sum(r["lookups"] for r in runs)
+    inserts = sum(r["inserts"] for r in runs)
+    launches = sum(r["hash_launches"] for r in runs)
+    new_per_s = inserts / t_exp
+    algo_bytes = BYTES_PER_FINDORPUT * (lookups + inserts)
+    achieved = algo_bytes / t_hash / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        o = oracle.explore(path, oracle.PROP_NONE, max_states=args.cpu_sample_states)
+        cpu = {
+            "value": o["states"] / o["seconds"],
+            "unit": "new states/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"oracle BFS of the same model, first {o['levels']} levels "
+                       f"({o['states']} states, {o['transitions']} transitions) in {o['seconds']:.1f} s"),
+        }
+
+    line = {
+        "metric": "new states/sec + hash-phase HBM GB/s vs peak, 1 GPU; batched vs plain",
+        "value": new_per_s,
+        "unit": "new states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_exp / args.steps,
+        "ms_per_step_wall": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "config2: ring of 20 LTSs x 16 states, 80-bit vectors, plain method, no property",
+            "generator": f"gpuexplore_amd.gen.config2(seed={args.seed})",
+            "table_ints": 1 << args.q_log2,
+            "states": states,
+            "transitions": runs[0]["transitions"],
+            "levels": runs[0]["levels"],
+            "key_words": info["key_words"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": None,
+            "kernel": "k_insert (FIND-OR-PUT)",
+            "algorithmic_bytes_per_launch": algo_bytes / launches,
+            "avg_launch_us": 1e6 * t_hash / launches,
+            "measured_copy_peak_gbps": peak.value,
+        },
+        "hash_phase": {
+            "lookups_per_s": lookups / t_hash,
+            "inserts_per_s": inserts / t_hash,
+            "t_hash_s_per_step": t_hash / args.steps,
+            "t_expand_s_per_step": sum(r["t_expand_s"] for r in runs) / args.steps,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

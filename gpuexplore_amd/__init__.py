@@ -44,6 +44,7 @@ class Options(ctypes.Structure):
         ("hash_seed", ctypes.c_uint64),
         ("cand_capacity", ctypes.c_uint64),
         ("frontier_capacity", ctypes.c_uint64),
+        ("max_states", ctypes.c_uint64),
     ]
 
 
@@ -58,6 +59,8 @@ class Result(ctypes.Structure):
         ("table_bytes", ctypes.c_uint64),
         ("violation", ctypes.c_int32),
         ("table_full", ctypes.c_int32),
+        ("truncated", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("t_explore_s", ctypes.c_double),
         ("t_hash_s", ctypes.c_double),
         ("t_expand_s", ctypes.c_double),
@@ -84,6 +87,9 @@ SIGNATURES = {
     "gx_options_default": (None, [ctypes.POINTER(Options)]),
     "gx_explore": (ctypes.c_int, [_P, ctypes.POINTER(Options), ctypes.POINTER(Result)]),
     "gx_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "gx_explorer_create": (ctypes.c_int, [_P, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
+    "gx_explorer_run": (ctypes.c_int, [_P, ctypes.POINTER(Result)]),
+    "gx_explorer_free": (None, [_P]),
     "gx_table_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                        ctypes.POINTER(_P)]),
     "gx_table_free": (None, [_P]),
@@ -177,6 +183,31 @@ def explore(model: Model, **kw) -> dict:
     if rc not in (GX_OK, -10):
         raise GxError(rc, "explore")
     return d
+
+
+class Explorer:
+    """gx_explorer: device buffers allocated once, reused by every run."""
+
+    def __init__(self, model: Model, **kw):
+        self.model = model  # keeps the model handle alive
+        self.opts = options(**kw)
+        h = ctypes.c_void_p()
+        _check(lib().gx_explorer_create(model.h, ctypes.byref(self.opts), ctypes.byref(h)), "explorer")
+        self.h = h
+
+    def run(self) -> dict:
+        r = Result()
+        rc = lib().gx_explorer_run(self.h, ctypes.byref(r))
+        d = r.as_dict()
+        d["status"] = rc
+        if rc not in (GX_OK, -10):
+            raise GxError(rc, "explorer run")
+        return d
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.gx_explorer_free(self.h)
+            self.h = None
 
 
 class Table:

@@ -73,7 +73,8 @@ struct Ctrl {
   uint32_t stop;            // no further level runs
   uint32_t done;            // frontier empty
   uint32_t levels_done;     // levels completed
-  uint32_t pad[2];
+  uint32_t truncated;       // stopped by max_states
+  uint64_t max_states;      // stop after the level that reaches this many states (0: no bound)
 };
 
 }  // namespace gx

@@ -101,14 +101,41 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Minimum over the N lanes of one process group: rotate-and-min with
-// distances 1, 2, 4, ... (covers any N, not only powers of two).
-__device__ __forceinline__ uint32_t group_min(uint32_t v, int N, int base, int p, bool in_group) {
-  for (int j = 1; j < N; j <<= 1) {
-    int src = in_group ? base + ((p + j) % N) : lane_id();
-    uint32_t o = __shfl(v, src, 64);
-    v = o < v ? o : v;
+// Reductions over the N lanes of one process group: rotate-and-combine
+// with distances 1, 2, 4, ... (covers any N, not only powers of two).  The
+// source lanes of the rotations are computed once per kernel.
+struct GroupRot {
+  int src[5];
+  int steps;
+};
+
+__device__ __forceinline__ GroupRot make_rot(int N, int base, int p, bool in_group) {
+  GroupRot r;
+  r.steps = 0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    const int j = 1 << i;
+    r.src[i] = in_group ? base + ((p + j) % N) : lane_id();
+    if (j < N) r.steps = i + 1;
   }
+  return r;
+}
+
+__device__ __forceinline__ uint32_t group_min(uint32_t v, const GroupRot &r) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    if (i < r.steps) {
+      const uint32_t o = __shfl(v, r.src[i], 64);
+      v = o < v ? o : v;
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t group_or(uint32_t v, const GroupRot &r) {
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+    if (i < r.steps) v |= __shfl(v, r.src[i], 64);
   return v;
 }
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -148,153 +149,177 @@ extern "C" const char *gx_strerror(int s) {
 }
 
 // ---------------------------------------------------------------------------
-// Single-model exploration (cu:1989-2142, 2168-2172).
-extern "C" int gx_explore(const gx_model *m, const gx_options *opt, gx_result *res) {
-  if (!m || !opt || !res) return GX_E_ARG;
-  memset(res, 0, sizeof *res);
+// Single-model exploration (cu:1989-2142, 2168-2172).  An explorer owns the
+// device buffers of one model and configuration; every run resets the
+// visited set and explores from the initial state.
+struct gx_explorer {
+  const gx_model *m = nullptr;
+  gx_options opt{};
+  int W = 1, WT = 1, BW = 16, mode = kModePlain;
+  bool stamp = false, lds = false;
+  uint32_t threads = 256, blocks = 1, cus = 1;
+  uint64_t nbuckets = 0, table_bytes = 0, front_cap = 0, cand_cap = 0;
+  DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
+  hipStream_t st = nullptr;
+  Events evs;
+  ~gx_explorer() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_explorer **out) {
+  if (!m || !opt || !out) return GX_E_ARG;
   if (device_count() == 0) return GX_E_NODEVICE;
-  const auto t_init0 = std::chrono::steady_clock::now();
   const ModelHdr &H = m->comp.hdr;
   const int W = static_cast<int>(H.key_words);
   const bool proviso = opt->por && opt->cycle_proviso;
-  const bool stamp = proviso;
   const uint32_t K = H.state_bits;
-  const int WT = stamp ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
+  const int WT = proviso ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
   if (WT > kMaxKeyWords || W > kMaxKeyWords) return GX_E_LIMIT;
   const uint32_t bb = opt->bucket_bytes ? opt->bucket_bytes : 128;
   if (bb != 64 && bb != 128 && bb != 256) return GX_E_ARG;
   const int BW = static_cast<int>(bb / 8);
   if (BW / WT < 1) return GX_E_ARG;
-  const uint32_t threads = round_threads(opt->threads);
+  gx_explorer *e = new (std::nothrow) gx_explorer();
+  if (!e) return GX_E_NOMEM;
+  std::unique_ptr<gx_explorer> guard(e);
+  e->m = m;
+  e->opt = *opt;
+  e->W = W;
+  e->WT = WT;
+  e->BW = BW;
+  e->stamp = proviso;
+  e->mode = opt->por ? (proviso ? kModePorProviso : kModePor) : kModePlain;
+  e->threads = round_threads(opt->threads);
   int dev = 0;
   GX_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   GX_TRY(hipGetDeviceProperties(&prop, dev));
-  const uint32_t cus = static_cast<uint32_t>(prop.multiProcessorCount);
-  const uint32_t blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : cus * (2048 / threads);
-
+  e->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+  e->blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : e->cus * (2048 / e->threads);
   size_t free_b = 0, total_b = 0;
   GX_TRY(hipMemGetInfo(&free_b, &total_b));
   // -q in 32-bit ints (cu:1838-1842); the default is a quarter of free
   // memory, at most 8 GiB (the reference takes all of it, cu:2054-2066)
-  uint64_t table_bytes = opt->q_ints ? opt->q_ints * 4ull
-                                     : std::min<uint64_t>(free_b / 4, 8ull << 30);
-  uint64_t nbuckets = table_bytes / bb;
-  if (nbuckets == 0) return GX_E_ARG;
-  table_bytes = nbuckets * bb;
-  const uint64_t slots = nbuckets * static_cast<uint64_t>(BW / WT);
-
-  DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
-  GX_TRY(d_table.alloc(table_bytes));
+  uint64_t table_bytes = opt->q_ints ? opt->q_ints * 4ull : std::min<uint64_t>(free_b / 4, 8ull << 30);
+  e->nbuckets = table_bytes / bb;
+  if (e->nbuckets == 0) return GX_E_ARG;
+  e->table_bytes = e->nbuckets * bb;
+  const uint64_t slots = e->nbuckets * static_cast<uint64_t>(BW / WT);
+  GX_TRY(e->d_table.alloc(e->table_bytes));
   GX_TRY(hipMemGetInfo(&free_b, &total_b));
   // buffers from what is left: frontiers are bounded by the slot count
   const uint64_t per_front = static_cast<uint64_t>(W) * 8ull + 8ull;  // key + retry index
   uint64_t front_cap = opt->frontier_capacity;
   if (!front_cap) front_cap = std::min<uint64_t>(slots, (free_b / 4) / (2 * per_front));
-  front_cap = std::max<uint64_t>(front_cap, 64);
+  e->front_cap = std::max<uint64_t>(front_cap, 64);
   uint64_t cand_cap = opt->cand_capacity;
-  if (!cand_cap)
-    cand_cap = std::min<uint64_t>(std::max<uint64_t>(4 * slots, 1u << 20), (free_b / 4) / (W * 8ull));
-  cand_cap = std::max<uint64_t>(cand_cap, 64);
-  GX_TRY(d_front0.alloc(front_cap * W * 8));
-  GX_TRY(d_front1.alloc(front_cap * W * 8));
-  GX_TRY(d_retry0.alloc(front_cap * 8));
-  GX_TRY(d_retry1.alloc(front_cap * 8));
-  GX_TRY(d_cand.alloc(cand_cap * W * 8));
-  GX_TRY(d_ctrl.alloc(sizeof(Ctrl)));
-  GX_TRY(d_model.alloc(m->comp.words.size() * 4));
-  GX_TRY(d_count.alloc(8));
+  if (!cand_cap) cand_cap = std::min<uint64_t>(std::max<uint64_t>(4 * slots, 1u << 20), (free_b / 4) / (W * 8ull));
+  e->cand_cap = std::max<uint64_t>(std::min<uint64_t>(cand_cap, 0xFFFFFFFFull), 64);
+  GX_TRY(e->d_front0.alloc(e->front_cap * W * 8));
+  GX_TRY(e->d_front1.alloc(e->front_cap * W * 8));
+  GX_TRY(e->d_retry0.alloc(e->front_cap * 8));
+  GX_TRY(e->d_retry1.alloc(e->front_cap * 8));
+  GX_TRY(e->d_cand.alloc(e->cand_cap * W * 8));
+  GX_TRY(e->d_ctrl.alloc(sizeof(Ctrl)));
+  GX_TRY(e->d_model.alloc(m->comp.words.size() * 4));
+  GX_TRY(e->d_count.alloc(8));
+  GX_TRY(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+  GX_TRY(hipMemcpyAsync(e->d_model.p, m->comp.words.data(), m->comp.words.size() * 4, hipMemcpyHostToDevice,
+                        e->st));
+  GX_TRY(hipStreamSynchronize(e->st));
+  const size_t lds_limit = 48 * 1024;
+  e->lds = static_cast<size_t>(H.total_words) * 4 + expand_scratch_bytes(e->threads) <= lds_limit;
+  *out = guard.release();
+  return GX_OK;
+}
 
-  hipStream_t st;
-  GX_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
-  } sg{st};
+extern "C" void gx_explorer_free(gx_explorer *e) { delete e; }
 
-  GX_TRY(hipMemcpyAsync(d_model.p, m->comp.words.data(), m->comp.words.size() * 4, hipMemcpyHostToDevice, st));
-  GX_TRY(hipMemsetAsync(d_table.p, 0, table_bytes, st));  // cu:656-663 init_queue
+extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
+  if (!e || !res) return GX_E_ARG;
+  memset(res, 0, sizeof *res);
+  const auto t_init0 = std::chrono::steady_clock::now();
+  const gx_options *opt = &e->opt;
+  const ModelHdr &H = e->m->comp.hdr;
+  const int W = e->W, WT = e->WT, BW = e->BW;
+  hipStream_t st = e->st;
+  GX_TRY(hipMemsetAsync(e->d_table.p, 0, e->table_bytes, st));  // cu:656-663 init_queue
   Ctrl c0{};
   c0.cur = 1;
   c0.level = 0xFFFFFFFFu;
   c0.levels_done = 0xFFFFFFFFu;
   c0.cand_count = 1;
-  GX_TRY(hipMemcpyAsync(d_ctrl.p, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  c0.max_states = opt->max_states;
+  GX_TRY(hipMemcpyAsync(e->d_ctrl.p, &c0, sizeof c0, hipMemcpyHostToDevice, st));
   std::vector<uint64_t> s0(W, kFlag);  // all processes in local state 0 (cu:668-682)
-  GX_TRY(hipMemcpyAsync(d_cand.p, s0.data(), W * 8, hipMemcpyHostToDevice, st));
+  GX_TRY(hipMemcpyAsync(e->d_cand.p, s0.data(), W * 8, hipMemcpyHostToDevice, st));
 
-  Table T{d_table.as<uint64_t>(), nbuckets, opt->hash_seed};
+  Table T{e->d_table.as<uint64_t>(), e->nbuckets, opt->hash_seed};
   ExpandArgs ea{};
-  ea.model = d_model.as<uint32_t>();
+  ea.model = e->d_model.as<uint32_t>();
   ea.hdr = H;
-  ea.ctrl = d_ctrl.as<Ctrl>();
-  ea.front[0] = d_front0.as<uint64_t>();
-  ea.front[1] = d_front1.as<uint64_t>();
-  ea.cand = d_cand.as<uint64_t>();
-  ea.cand_cap = cand_cap;
-  ea.retry[0] = d_retry0.as<uint64_t>();
-  ea.retry[1] = d_retry1.as<uint64_t>();
+  ea.ctrl = e->d_ctrl.as<Ctrl>();
+  ea.front[0] = e->d_front0.as<uint64_t>();
+  ea.front[1] = e->d_front1.as<uint64_t>();
+  ea.cand = e->d_cand.as<uint64_t>();
+  ea.cand_cap = e->cand_cap;
+  ea.retry[0] = e->d_retry0.as<uint64_t>();
+  ea.retry[1] = e->d_retry1.as<uint64_t>();
   ea.property = opt->property;
   ea.bucket_words = BW;
   ea.table = T;
   InsertArgs ia{};
-  ia.ctrl = d_ctrl.as<Ctrl>();
-  ia.cand = d_cand.as<uint64_t>();
-  ia.front[0] = d_front0.as<uint64_t>();
-  ia.front[1] = d_front1.as<uint64_t>();
-  ia.front_cap = front_cap;
+  ia.ctrl = e->d_ctrl.as<Ctrl>();
+  ia.cand = e->d_cand.as<uint64_t>();
+  ia.cand_cap = e->cand_cap;
+  ia.front[0] = e->d_front0.as<uint64_t>();
+  ia.front[1] = e->d_front1.as<uint64_t>();
+  ia.front_cap = e->front_cap;
   ia.table = T;
-  ia.state_bits = K;
+  ia.state_bits = H.state_bits;
   ia.is_new = nullptr;
   ia.append = 1;
 
-  const int mode = opt->por ? (proviso ? kModePorProviso : kModePor) : kModePlain;
-  const size_t lds_limit = 48 * 1024;
-  const bool lds = static_cast<size_t>(H.total_words) * 4 + expand_scratch_bytes(threads) <= lds_limit;
-  if (!lds) {
-    // scratch only
-  }
-
   // initial state: FIND-OR-PUT of s0 into frontier 0 with stamp 0
-  GX_TRY(launch_insert(ia, W, WT, BW, stamp, 1, 64, st));
+  GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, 1, 64, st));
   GX_TRY(launch_advance(ia.ctrl, st));
   GX_TRY(hipStreamSynchronize(st));
   const auto t_init1 = std::chrono::steady_clock::now();
   res->t_init_s = std::chrono::duration<double>(t_init1 - t_init0).count();
 
   const uint32_t kiters = std::max<uint32_t>(1, opt->kernel_iters);
-  Events evs;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ins_ev, exp_ev;
   double hash_ms = 0, exp_ms = 0;
   Ctrl ch{};
   const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
-    evs.used = 0;
+    e->evs.used = 0;
     ins_ev.clear();
     exp_ev.clear();
     for (uint32_t it = 0; it < kiters; it++) {
-      hipEvent_t a0 = evs.next(), a1 = evs.next(), b0 = evs.next(), b1 = evs.next();
+      hipEvent_t a0 = e->evs.next(), a1 = e->evs.next(), b0 = e->evs.next(), b1 = e->evs.next();
       if (!a0 || !a1 || !b0 || !b1) return GX_E_HIP;
       GX_TRY(hipEventRecord(a0, st));
-      GX_TRY(launch_expand(ea, mode, W, WT, lds, blocks, threads, st));
+      GX_TRY(launch_expand(ea, e->mode, W, WT, e->lds, e->blocks, e->threads, st));
       GX_TRY(hipEventRecord(a1, st));
       GX_TRY(hipEventRecord(b0, st));
-      GX_TRY(launch_insert(ia, W, WT, BW, stamp, blocks, threads, st));
+      GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, e->blocks, e->threads, st));
       GX_TRY(hipEventRecord(b1, st));
       GX_TRY(launch_advance(ia.ctrl, st));
       exp_ev.push_back({a0, a1});
       ins_ev.push_back({b0, b1});
     }
-    GX_TRY(hipMemcpyAsync(&ch, d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
+    GX_TRY(hipMemcpyAsync(&ch, e->d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
     GX_TRY(hipStreamSynchronize(st));
-    for (auto &e : exp_ev) {
+    for (auto &p : exp_ev) {
       float ms = 0;
-      GX_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+      GX_TRY(hipEventElapsedTime(&ms, p.first, p.second));
       exp_ms += ms;
     }
-    for (auto &e : ins_ev) {
+    for (auto &p : ins_ev) {
       float ms = 0;
-      GX_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+      GX_TRY(hipEventElapsedTime(&ms, p.first, p.second));
       hash_ms += ms;
     }
     res->hash_launches += ins_ev.size();
@@ -306,10 +331,10 @@ extern "C" int gx_explore(const gx_model *m, const gx_options *opt, gx_result *r
   res->t_hash_s = hash_ms / 1e3;
   res->t_expand_s = exp_ms / 1e3;
 
-  GX_TRY(hipMemsetAsync(d_count.p, 0, 8, st));
-  GX_TRY(launch_count(T, WT, BW, d_count.as<uint64_t>(), cus * 8, st));
+  GX_TRY(hipMemsetAsync(e->d_count.p, 0, 8, st));
+  GX_TRY(launch_count(T, WT, BW, e->d_count.as<uint64_t>(), e->cus * 8, st));
   uint64_t counted = 0;
-  GX_TRY(hipMemcpyAsync(&counted, d_count.p, 8, hipMemcpyDeviceToHost, st));
+  GX_TRY(hipMemcpyAsync(&counted, e->d_count.p, 8, hipMemcpyDeviceToHost, st));
   GX_TRY(hipStreamSynchronize(st));
 
   res->states = counted;
@@ -319,11 +344,26 @@ extern "C" int gx_explore(const gx_model *m, const gx_options *opt, gx_result *r
   res->inserts = ch.inserts;
   res->violation = ch.violation ? 1 : 0;
   res->table_full = ch.table_full ? 1 : 0;
-  res->table_bytes = table_bytes;
+  res->truncated = ch.truncated ? 1 : 0;
+  res->table_bytes = e->table_bytes;
   if (ch.bad_state) return GX_E_STATE;
   if (ch.overflow) return GX_E_CAPACITY;
   if (ch.spin_timeout) return GX_E_HIP;
   return GX_OK;
+}
+
+extern "C" int gx_explore(const gx_model *m, const gx_options *opt, gx_result *res) {
+  if (!m || !opt || !res) return GX_E_ARG;
+  memset(res, 0, sizeof *res);
+  const auto t0 = std::chrono::steady_clock::now();
+  gx_explorer *e = nullptr;
+  int rc = gx_explorer_create(m, opt, &e);
+  if (rc != GX_OK) return rc;
+  const double t_alloc = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  rc = gx_explorer_run(e, res);
+  res->t_init_s += t_alloc;
+  gx_explorer_free(e);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -377,6 +417,7 @@ static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8
   InsertArgs ia{};
   ia.ctrl = t->ctrl.as<Ctrl>();
   ia.cand = dev_keys;
+  ia.cand_cap = n;
   ia.table = Table{t->words.as<uint64_t>(), t->nbuckets, t->seed};
   ia.state_bits = 63 * t->W;
   ia.is_new = dev_status;

@@ -25,6 +25,7 @@ struct ExpandArgs {
 struct InsertArgs {
   Ctrl *ctrl;
   const uint64_t *cand;
+  uint64_t cand_cap;
   uint64_t *front[2];
   uint64_t front_cap;
   Table table;

@@ -14,6 +14,8 @@
 namespace gx {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic (expand)
+constexpr int kStage = 128;        // new keys staged per wave before a flush (insert)
 
 // ---------------------------------------------------------------------------
 // Successor generation.
@@ -66,6 +68,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   const bool in_group = g < G;
   const int base = in_group ? g * N : 0;
   const uint64_t nmask = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
+  const GroupRot rot = make_rot(N, base, p, in_group);
 
   const uint32_t floc = in_group ? M[H.off_floc + p] : 0u;
   const uint32_t rowbase = in_group ? M[H.off_rowbase + p] : 0u;
@@ -84,6 +87,13 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint64_t acc_trans = 0, acc_states = 0;
   bool viol = false, bad = false;
+  // this wave's current candidate chunk
+  uint64_t cbase = 0, cleft = 0;
+  bool cfull = false;
+  auto release = [&](uint64_t b, uint64_t n) {
+    const uint64_t end = (b + n < a.cand_cap) ? b + n : a.cand_cap;
+    for (uint64_t i = b + lane; i < end; i += 64) a.cand[i * W] = 0;
+  };
 
   for (uint64_t b0 = gw * G; b0 < nwork; b0 += nw * G) {
     const uint64_t si = b0 + g;
@@ -119,7 +129,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
       uint32_t e = e0;
       for (;;) {
         const uint32_t mylab = (e < e1) ? M[H.off_syn_lab + e] : kNone;
-        const uint32_t curl = group_min(mylab, N, base, p, in_group);
+        const uint32_t curl = group_min(mylab, rot);
         if (!ballot(in_group && curl != kNone)) break;
         const bool mine = in_group && mylab == curl && curl != kNone;
         const uint64_t en = (ballot(mine) >> base) & nmask;
@@ -178,17 +188,18 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
           if (safety && ((r >> (N - 1)) & 1u) && last_hits_one()) viol = true;
           if (MODE == kModePorProviso) {
             // enumerate until a successor is new (not visited at level <= L)
-            const uint64_t prod = rule_product(r);
+            const uint64_t prod64 = rule_product(r);
+            const uint32_t prod = prod64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(prod64);
             bool found_new = false;
-            for (uint64_t k = 0; k < prod && !found_new; k++) {
+            for (uint32_t k = 0; k < prod && !found_new; k++) {
               uint64_t s[W];
 #pragma unroll
               for (int j = 0; j < W; j++) s[j] = key[j];
-              uint64_t rem = k;
+              uint32_t rem = k;
               for (uint32_t rr = r; rr; rr &= rr - 1) {
                 const int q = __ffs(rr) - 1;
                 const uint2 t = xch[base + q];
-                const uint32_t d = static_cast<uint32_t>(rem % t.y);
+                const uint32_t d = rem % t.y;
                 rem /= t.y;
                 kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
               }
@@ -228,11 +239,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
           if (!f || stamp > level) local_new = true;
         }
         // OR of the new-successor rules over the group
-        uint32_t acc = rnew;
-        for (int j = 1; j < N; j <<= 1) {
-          const int src = in_group ? base + ((p + j) % N) : lane;
-          acc |= __shfl(acc, src, 64);
-        }
+        const uint32_t acc = group_or(rnew, rot);
         prov = local_new || ((acc >> p) & 1u);
       }
       const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
@@ -277,29 +284,41 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
     uint64_t obase = 0;
     int ok = 1;
     if (total) {
-      if (lane == 0) {
-        if (total > a.cand_cap) {
-          ok = -1;
+      // Candidate space is reserved in chunks: one atomicAdd per chunk, sized
+      // for the wave's remaining batches (at most kChunk entries unless one
+      // batch needs more).  Unused chunk tails are released by zero-filling
+      // their first words (a zero first word is never a key: keys carry bit
+      // 63).  A chunk that overruns the buffer defers the wave's batches.
+      if (cfull) {
+        ok = 0;
+      } else if (total > cleft) {
+        release(cbase, cleft);
+        const uint64_t left = (nwork - b0 + nw * G - 1) / (nw * G);
+        uint64_t want = total * left;
+        want = want < kChunk ? want : kChunk;
+        want = want > total ? want : total;
+        uint64_t got = 0;
+        if (lane == 0)
+          got = atomicAdd(reinterpret_cast<unsigned long long *>(&C->cand_count),
+                          static_cast<unsigned long long>(want));
+        got = __shfl(got, 0, 64);
+        const uint64_t avail = got < a.cand_cap ? a.cand_cap - got : 0;
+        if (avail >= total) {
+          cbase = got;
+          cleft = want < avail ? want : avail;
         } else {
-          uint64_t old = __hip_atomic_load(&C->cand_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          for (;;) {
-            if (old + total > a.cand_cap) {
-              ok = 0;
-              break;
-            }
-            const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long *>(&C->cand_count),
-                                            static_cast<unsigned long long>(old),
-                                            static_cast<unsigned long long>(old + total));
-            if (prev == old) {
-              obase = old;
-              break;
-            }
-            old = prev;
-          }
+          release(got, want);
+          cbase = 0;
+          cleft = 0;
+          cfull = true;
+          ok = (total > a.cand_cap) ? -1 : 0;
         }
       }
-      ok = __shfl(ok, 0, 64);
-      obase = __shfl(obase, 0, 64);
+      if (ok > 0) {
+        obase = cbase;
+        cbase += total;
+        cleft -= total;
+      }
     }
     if (ok <= 0) {
       // defer this wave's states to the retry list
@@ -332,16 +351,17 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
     walk([&](uint32_t, uint64_t, bool) {},
          [&](uint32_t r, uint32_t) {
            if (!has || (r & ~csel) != 0) return;
-           const uint64_t prod = rule_product(r);
-           for (uint64_t k = 0; k < prod; k++) {
+           // emitted products fit the candidate buffer, so < 2^32
+           const uint32_t prod = static_cast<uint32_t>(rule_product(r));
+           for (uint32_t k = 0; k < prod; k++) {
              uint64_t s[W];
 #pragma unroll
              for (int j = 0; j < W; j++) s[j] = key[j];
-             uint64_t rem = k;
+             uint32_t rem = k;
              for (uint32_t rr = r; rr; rr &= rr - 1) {
                const int q = __ffs(rr) - 1;
                const uint2 t = xch[base + q];
-               const uint32_t d = static_cast<uint32_t>(rem % t.y);
+               const uint32_t d = rem % t.y;
                rem /= t.y;
                kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
              }
@@ -351,6 +371,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
            }
          });
   }
+  release(cbase, cleft);
   if (lane == 0) {
     if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
                              static_cast<unsigned long long>(acc_trans));
@@ -362,13 +383,16 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// FIND-OR-PUT over the candidate buffer; new states are appended to the
-// next frontier with one atomic per wave and a ballot rank.
+// FIND-OR-PUT over the candidate buffer.  New keys are staged per wave in
+// LDS and appended to the next frontier in blocks of up to kStage keys, one
+// atomicAdd per block: a single contended counter would otherwise serialise
+// all waves of the device.
 template <int W, int WT, int BW, bool STAMP>
 __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
+  extern __shared__ uint64_t stage_all[];
   Ctrl *C = a.ctrl;
   if (C->stop) return;
-  const uint64_t n = C->cand_count;
+  const uint64_t n = C->cand_count < a.cand_cap ? C->cand_count : a.cand_cap;
   const uint32_t nxt = C->cur ^ 1u;
   uint64_t *out = a.front[nxt];
   unsigned long long *out_count = reinterpret_cast<unsigned long long *>(&C->front_count[nxt]);
@@ -376,53 +400,67 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
   make_fmt<WT, STAMP>(fmt, a.state_bits, C->level + 1u);
   constexpr int LPB = BW / 2;
   constexpr int CPW = 64 / LPB;
+  static_assert(CPW <= kStage, "stage smaller than one iteration");
   const int lane = lane_id();
   const int sub = lane % LPB;
   const int cg = lane / LPB;
   const int gbase = cg * LPB;
+  uint64_t *stage = stage_all + static_cast<size_t>(threadIdx.x >> 6) * (kStage * W);
+  uint32_t staged = 0;  // wave-uniform
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint64_t acc_new = 0;
+  uint64_t acc_new = 0, acc_look = 0;
   bool full = false, timeout = false, overflow = false;
+  auto flush = [&]() {
+    if (!staged) return;
+    wave_sync();
+    uint64_t ob = 0;
+    if (lane == 0) ob = atomicAdd(out_count, static_cast<unsigned long long>(staged));
+    ob = __shfl(ob, 0, 64);
+    for (uint32_t i = lane; i < staged * W; i += 64) {
+      if (ob + i / W < a.front_cap)
+        out[ob * W + i] = stage[i];
+      else
+        overflow = true;
+    }
+    wave_sync();
+    staged = 0;
+  };
   for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
     const uint64_t ci = c0 + cg;
-    const bool active = ci < n;
+    bool active = ci < n;
     uint64_t key[W];
 #pragma unroll
     for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
+    active = active && key[0] != 0;  // released reservation
     uint64_t v[WT];
     slot_value<W, WT>(key, fmt, v);
     const uint64_t h = key_hash<W>(key, a.table.seed);
     const int st = find_or_put_group<WT, BW>(a.table, v, fmt, h, sub, gbase, active);
+    acc_look += __popcll(ballot(active && sub == 0));
     const bool is_new = active && st == kInserted;
     full |= active && st == kFull;
     timeout |= active && st == kTimeout;
     if (a.is_new && active && sub == 0) a.is_new[ci] = is_new ? 1 : 0;
     const uint64_t m = ballot(is_new && sub == 0);
-    if (m && !a.append) {
-      acc_new += __popcll(m);
-    } else if (m) {
-      const uint32_t cnt = __popcll(m);
-      uint64_t ob = 0;
-      if (lane == 0) ob = atomicAdd(out_count, static_cast<unsigned long long>(cnt));
-      ob = __shfl(ob, 0, 64);
-      if (is_new) {
-        const uint64_t idx = ob + __popcll(m & ((1ull << gbase) - 1ull));
-        if (idx < a.front_cap) {
+    if (!m) continue;
+    const uint32_t cnt = __popcll(m);
+    acc_new += cnt;
+    if (!a.append) continue;
+    if (staged + cnt > static_cast<uint32_t>(kStage)) flush();
+    if (is_new) {
+      const uint32_t pos = staged + __popcll(m & ((1ull << gbase) - 1ull));
 #pragma unroll
-          for (int j = 0; j < W; j++)
-            if (j % LPB == sub) out[idx * W + j] = key[j];
-        } else {
-          overflow = true;
-        }
-      }
-      acc_new += cnt;
+      for (int j = 0; j < W; j++)
+        if (j % LPB == sub) stage[pos * W + j] = key[j];
     }
+    staged += cnt;
   }
+  if (a.append) flush();
   if (lane == 0 && acc_new)
     atomicAdd(reinterpret_cast<unsigned long long *>(&C->inserts), static_cast<unsigned long long>(acc_new));
-  if (blockIdx.x == 0 && threadIdx.x == 0 && n)
-    atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(n));
+  if (lane == 0 && acc_look)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(acc_look));
   if (ballot(full) && lane == 0) C->table_full = 1;
   if (ballot(timeout) && lane == 0) C->spin_timeout = 1;
   if (ballot(overflow) && lane == 0) C->overflow = 1;
@@ -455,6 +493,10 @@ __global__ void k_advance(Ctrl *c) {
   c->level += 1u;
   c->levels_done += 1u;
   if (c->violation) c->stop = 1;
+  if (c->max_states && c->inserts >= c->max_states && c->front_count[c->cur] != 0) {
+    c->truncated = 1;
+    c->stop = 1;
+  }
   if (c->front_count[c->cur] == 0) {
     c->done = 1;
     c->stop = 1;
@@ -524,12 +566,13 @@ hipError_t launch_expand(const ExpandArgs &a, int mode, int key_words, int slot_
 
 template <int W, int WT, bool STAMP>
 static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  const size_t shm = static_cast<size_t>(threads / 64) * kStage * W * 8;
   switch (bw) {
     case 8:
-      if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break; }
+      if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break; }
       return hipErrorInvalidValue;
-    case 16: hipLaunchKernelGGL((k_insert<W, WT, 16, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_insert<W, WT, 32, STAMP>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_insert<W, WT, 16, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break;
+    case 32: hipLaunchKernelGGL((k_insert<W, WT, 32, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

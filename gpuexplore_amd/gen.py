@@ -307,6 +307,71 @@ def throttled(n: int, nstates: int, tokens: int, outdeg: int,
     return net
 
 
+def rounds(n: int, nstates: int, tokens: int, budgets: Sequence[int], n_local_labels: int,
+           seed: int, monitor: Tuple[int, int, int, int] | None = None,
+           mix: Tuple[float, float, float] | None = None, outdeg: int = 3,
+           phased: bool = False) -> Network:
+    """Ring of ``n`` seeded random LTSs that run in rounds.
+
+    Local state ``c * tokens + u`` pairs a control state ``c`` with ``u``
+    tokens spent.  Every control state has out-degree 3: one local move
+    (random local label, random target), one rendezvous with a neighbour
+    (random target) and one n-way barrier; with ``mix = (local,
+    rendezvous, barrier)`` it has ``outdeg`` transitions whose kinds are drawn
+    from the mix instead.  With ``phased`` the round has two phases: local
+    moves start from unspent states only, while rendezvous and barrier moves
+    start from spent states only, so states at the start of a round carry no
+    synchronising labels (the situation cluster-based POR reduces).  Local and rendezvous moves need
+    and spend a token (process p may spend ``budgets[p] <= tokens-1`` per
+    round); the barrier sends every process home to state 0.  The reachable
+    set is therefore what one round can reach, and the per-process budgets
+    set its size.
+    """
+    if nstates % tokens or len(budgets) != n or max(budgets) >= tokens:
+        raise ValueError("bad rounds parameters")
+    ctrl = nstates // tokens
+    rng = Rng(seed)
+    L = n_local_labels
+    RDV, BAR = L, L + 1
+    sizes = [nstates] * n + ([2] if monitor else [])
+    local, sync = _empty(sizes)
+    for p in range(n):
+        for c in range(ctrl):
+            if mix is None:
+                kinds = [0, 1, 2]
+            else:
+                kinds = []
+                for _ in range(outdeg):
+                    x = rng.uniform()
+                    kinds.append(0 if x < mix[0] else (1 if x < mix[0] + mix[1] else 2))
+            for kind in kinds:
+                if kind == 0:
+                    lab, cl = rng.below(L), rng.below(ctrl)
+                    for u in range(budgets[p]):
+                        _add_local(local, p, c * tokens + u, lab, cl * tokens + u + 1)
+                elif kind == 1:
+                    cr = rng.below(ctrl)
+                    lo = 1 if phased else 0
+                    for u in range(lo, budgets[p] + lo):
+                        _add_sync(sync, p, c * tokens + u, RDV, cr * tokens + min(u + 1, tokens - 1))
+                else:
+                    for u in range(1 if phased else 0, tokens):
+                        _add_sync(sync, p, c * tokens + u, BAR, 0)
+    rules = {RDV: ring_rules(n) if n >= 3 else [3], BAR: [(1 << n) - 1]}
+    nlabels = L + 2
+    if monitor:
+        i, x, j, y = monitor
+        BAD = L + 2
+        nlabels = L + 3
+        _add_sync(sync, i, x, BAD, x)
+        _add_sync(sync, j, y, BAD, y)
+        _add_sync(sync, n, 0, BAD, 1)
+        rules[BAD] = [(1 << i) | (1 << j) | (1 << n)]
+    net = Network(sizes, nlabels, local, sync, rules)
+    net.check()
+    return net
+
+
 def config1(seed: int = 1) -> Network:
     """Ring of 6 processes, 8 states, out-degree 2, labels 0-2 local, label 3
     a 2-party rendezvous with the right neighbour; one 32-bit word."""
@@ -328,15 +393,24 @@ def config1(seed: int = 1) -> Network:
 
 
 def config2(seed: int = 1) -> Network:
-    """Ring of 20 processes, 16 states (4 control x 4 tokens), out-degree 3,
-    labels 0-3 local, 4 neighbour rendezvous, 5 global 20-way barrier."""
-    return throttled(20, 16, 4, 3, (0.70, 0.20, 0.10), 4, seed)
+    """Ring of 20 processes with 16 local states each (4 control states x a
+    token variable of domain 4: 4 bits per process, 80 bits, 3 words),
+    out-degree 3, labels 0-3 local, 4 neighbour rendezvous, 5 global 20-way
+    barrier.  Local and rendezvous moves spend tokens; the barrier sends
+    everyone home.  Processes 0, 5, 10, 15 may spend two tokens per round,
+    the others one."""
+    budgets = [2 if p % 5 == 0 else 1 for p in range(20)]
+    return rounds(20, 16, 4, budgets, 4, seed)
 
 
-def config3(seed: int = 1) -> Network:
-    """24 processes, 8 states (4 control x 2 tokens), 70% local, 20% neighbour
-    rendezvous, 10% barrier transitions (POR workload)."""
-    return throttled(24, 8, 2, 2, (0.70, 0.20, 0.10), 4, seed)
+def config3(seed: int = 3) -> Network:
+    """24 processes with 8 local states each (4 control states x 2 tokens
+    spent; 3 bits per process, 72 bits, 3 words), out-degree 3 drawn 70%
+    local, 20% neighbour rendezvous, 10% barrier.  Rounds are phased: local
+    moves from unspent states, rendezvous and barrier moves from spent
+    states, so a round starts with independent local moves only (the POR
+    workload)."""
+    return rounds(24, 8, 4, [1] * 24, 4, seed, mix=(0.7, 0.2, 0.1), phased=True)
 
 
 def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, str]]:

@@ -85,6 +85,7 @@ typedef struct gx_options {
   uint64_t hash_seed;        /* seeds the bucket hash; counts do not depend on it */
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
+  uint64_t max_states;       /* stop after the level that reaches this many states (0 = none) */
 } gx_options;
 
 void gx_options_default(gx_options *o);
@@ -99,6 +100,8 @@ typedef struct gx_result {
   uint64_t table_bytes;   /* bytes of the visited set */
   int32_t violation;      /* deadlock / safety violation found (cu:2150-2162) */
   int32_t table_full;     /* cu:2164-2166 "problem with hash table" */
+  int32_t truncated;      /* stopped by max_states */
+  int32_t reserved;
   double t_explore_s;     /* wall time of the exploration (excludes parse/alloc/init) */
   double t_hash_s;        /* summed FIND-OR-PUT kernel time (HIP events) */
   double t_expand_s;      /* summed successor-generation kernel time (HIP events) */
@@ -108,6 +111,15 @@ typedef struct gx_result {
 /* Explore the state space on the current HIP device: replaces the device
  * setup, BFS driver loop and count of main() (cu:1989-2142, 2168-2172). */
 int gx_explore(const gx_model *m, const gx_options *o, gx_result *r);
+
+/* Reusable exploration: create allocates the visited set (-q) and the
+ * frontier and successor buffers once; every run resets the visited set
+ * and explores from the initial state (cu:2091-2092 init, 2099-2142
+ * driver).  Runs of one explorer must not overlap. */
+typedef struct gx_explorer gx_explorer;
+int gx_explorer_create(const gx_model *m, const gx_options *o, gx_explorer **out);
+int gx_explorer_run(gx_explorer *e, gx_result *r);
+void gx_explorer_free(gx_explorer *e);
 
 const char *gx_strerror(int status);
 
