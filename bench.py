@@ -39,6 +39,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-states", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane")
+    ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -61,7 +63,8 @@ def main() -> int:
     peak = ctypes.c_double(0)
     gx.lib().gx_bench_copy(4 << 30, 20, ctypes.byref(peak))
 
-    ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE)
+    ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE,
+                     expand_kernel=args.expand_kernel, bucket_bytes=args.bucket_bytes)
     for i in range(args.warmup):
         r = ex.run()
         print(f"warmup {i}: states={r['states']} t_explore={r['t_explore_s']:.3f}s", file=sys.stderr, flush=True)
@@ -121,6 +124,8 @@ def main() -> int:
             "transitions": runs[0]["transitions"],
             "levels": runs[0]["levels"],
             "key_words": info["key_words"],
+            "bucket_bytes": args.bucket_bytes,
+            "expand_kernel": ["auto", "process-groups", "state-per-lane"][args.expand_kernel],
         },
         "roofline": {
             "bound": "hbm",

@@ -45,6 +45,8 @@ class Options(ctypes.Structure):
         ("cand_capacity", ctypes.c_uint64),
         ("frontier_capacity", ctypes.c_uint64),
         ("max_states", ctypes.c_uint64),
+        ("expand_kernel", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -26,7 +26,7 @@ static const char *kHelp =
     "  -b NUM             Run the kernels on NUM workgroups (default: fill the GPU)\n"
     "  -t NUM             Use NUM threads per workgroup, a multiple of 64 (default 256)\n"
     "  -q NUM             Allocate NUM integers for the hash table\n"
-    "  --bucket BYTES     Hash bucket width: 64, 128 or 256 (default 128)\n"
+    "  --bucket BYTES     Hash bucket width: 64, 128 or 256 (default 64 up to 126-bit states)\n"
     "  --seed NUM         Hash seed (state counts do not depend on it)\n"
     "  --dump FILE        Dump the state space to FILE after completing the exploration\n"
     "  -v NUM             Change the verbosity:\n"

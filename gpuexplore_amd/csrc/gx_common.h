@@ -43,6 +43,8 @@ struct ModelHdr {
   // per label
   uint32_t off_rule_start; // nlab + 1, CSR into rules
   uint32_t off_rules;      // N-bit process masks
+  uint32_t off_rowlab;     // nrows x 2 words: mask of the synchronising labels of the row (nlab <= 64)
+  uint32_t lab64;          // 1 when every label fits the 64-bit row masks
 };
 
 enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u };

@@ -127,7 +127,7 @@ extern "C" int gx_model_info_get(const gx_model *m, gx_model_info *o) {
 extern "C" void gx_options_default(gx_options *o) {
   memset(o, 0, sizeof *o);
   o->kernel_iters = 1;
-  o->bucket_bytes = 128;
+  o->bucket_bytes = 0;
   o->hash_seed = 0x5eed5eed5eedull;
 }
 
@@ -156,7 +156,7 @@ struct gx_explorer {
   const gx_model *m = nullptr;
   gx_options opt{};
   int W = 1, WT = 1, BW = 16, mode = kModePlain;
-  bool stamp = false, lds = false;
+  bool stamp = false, lds = false, per_state = false;
   uint32_t threads = 256, blocks = 1, cus = 1;
   uint64_t nbuckets = 0, table_bytes = 0, front_cap = 0, cand_cap = 0;
   DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
@@ -176,7 +176,9 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   const uint32_t K = H.state_bits;
   const int WT = proviso ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
   if (WT > kMaxKeyWords || W > kMaxKeyWords) return GX_E_LIMIT;
-  const uint32_t bb = opt->bucket_bytes ? opt->bucket_bytes : 128;
+  // bucket width: one 64-byte half line holds 4-8 slots of up to 2 words,
+  // wider slots get a full 128-byte line
+  const uint32_t bb = opt->bucket_bytes ? opt->bucket_bytes : (WT <= 2 ? 64u : 128u);
   if (bb != 64 && bb != 128 && bb != 256) return GX_E_ARG;
   const int BW = static_cast<int>(bb / 8);
   if (BW / WT < 1) return GX_E_ARG;
@@ -228,8 +230,17 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   GX_TRY(hipMemcpyAsync(e->d_model.p, m->comp.words.data(), m->comp.words.size() * 4, hipMemcpyHostToDevice,
                         e->st));
   GX_TRY(hipStreamSynchronize(e->st));
-  const size_t lds_limit = 48 * 1024;
-  e->lds = static_cast<size_t>(H.total_words) * 4 + expand_scratch_bytes(e->threads) <= lds_limit;
+  // successor kernel: one state per lane for plain exploration of models
+  // whose labels fit the 64-bit row masks, else the process-group kernel
+  const size_t lds_limit = 64 * 1024;
+  const size_t model_b = ((H.total_words + 3u) & ~3u) * 4u;
+  const size_t tps_b = expand_tps_scratch_bytes(e->threads, H.nprocs);
+  e->per_state = e->mode == kModePlain && H.lab64 && opt->expand_kernel != 1 && tps_b <= lds_limit;
+  if (opt->expand_kernel == 2 && !e->per_state) return GX_E_ARG;
+  if (e->per_state)
+    e->lds = model_b + tps_b <= lds_limit;
+  else
+    e->lds = model_b + expand_scratch_bytes(e->threads) <= 48 * 1024;
   *out = guard.release();
   return GX_OK;
 }
@@ -269,6 +280,7 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   ea.property = opt->property;
   ea.bucket_words = BW;
   ea.table = T;
+  ea.per_state = e->per_state ? 1 : 0;
   InsertArgs ia{};
   ia.ctrl = e->d_ctrl.as<Ctrl>();
   ia.cand = e->d_cand.as<uint64_t>();

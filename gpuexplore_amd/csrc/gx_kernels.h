@@ -20,7 +20,13 @@ struct ExpandArgs {
   int property;
   int bucket_words;  // FIND under the cycle proviso
   Table table;
+  int per_state;     // plain mode: one state per lane (k_expand_tps)
 };
+
+// LDS bytes k_expand_tps needs besides the staged model.
+inline size_t expand_tps_scratch_bytes(uint32_t threads, uint32_t nprocs) {
+  return static_cast<size_t>(threads / 64) * nprocs * 64 * 4;
+}
 
 struct InsertArgs {
   Ctrl *ctrl;

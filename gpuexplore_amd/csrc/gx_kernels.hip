@@ -383,6 +383,241 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Successor generation, one state per lane (plain mode, labels < 64).
+//
+// For models whose per-process work is small (a few entries per local
+// state) a lane expanding a whole state is cheaper than a group of N lanes:
+// 64 states per wave instead of floor(64/N), and no cross-lane traffic
+// except the wave's output reservation.  The per-process row of the state is
+// kept in LDS ([process][lane], conflict-free); the synchronising labels of a
+// row are a 64-bit mask, so En(a) is built per label and the target list of
+// process q for label a is entry popcount(mask_q & below(a)) of q's row.
+// Semantics are those of the group kernel above.
+template <int W, bool LDS>
+__global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
+  extern __shared__ uint32_t smem[];
+  const ModelHdr &H = a.hdr;
+  Ctrl *C = a.ctrl;
+  if (LDS) {
+    for (uint32_t i = threadIdx.x; i < H.total_words; i += blockDim.x) smem[i] = a.model[i];
+  }
+  const uint32_t *M = LDS ? smem : a.model;
+  const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
+  const int N = static_cast<int>(H.nprocs);
+  const int wib = threadIdx.x >> 6;
+  uint32_t *rows = smem + xoff + static_cast<uint32_t>(wib) * (N * 64u);
+  __syncthreads();
+  if (C->stop) return;
+  const uint32_t cur = C->cur;
+  const uint32_t use_retry = C->use_retry;
+  const uint32_t rc = C->retry_cur;
+  const uint64_t nwork = use_retry ? C->retry_count[rc] : C->front_count[cur];
+  const uint64_t *retry_in = a.retry[rc];
+  uint64_t *retry_out = a.retry[rc ^ 1u];
+  uint64_t *retry_out_count = &C->retry_count[rc ^ 1u];
+  const uint64_t *front = a.front[cur];
+  const int lane = lane_id();
+  const bool safety = a.property == 2;
+  const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
+  const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
+
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint64_t acc_trans = 0, acc_states = 0;
+  bool viol = false, bad = false;
+  uint64_t cbase = 0, cleft = 0;
+  bool cfull = false;
+  auto release = [&](uint64_t b, uint64_t n) {
+    const uint64_t end = (b + n < a.cand_cap) ? b + n : a.cand_cap;
+    for (uint64_t i = b + lane; i < end; i += 64) a.cand[i * W] = 0;
+  };
+  auto rowlab = [&](uint32_t row) -> uint64_t {
+    return static_cast<uint64_t>(M[H.off_rowlab + 2 * row]) |
+           (static_cast<uint64_t>(M[H.off_rowlab + 2 * row + 1]) << 32);
+  };
+  // entry of process q's row for label l (present in the row's mask)
+  auto syn_entry = [&](uint32_t row, uint32_t l) -> uint32_t {
+    const uint64_t below = (l >= 64) ? ~0ull : ((1ull << l) - 1ull);
+    return M[H.off_syn_start + row] + static_cast<uint32_t>(__popcll(rowlab(row) & below));
+  };
+
+  for (uint64_t b0 = gw * 64; b0 < nwork; b0 += nw * 64) {
+    const uint64_t si = b0 + lane;
+    const bool has = si < nwork;
+    uint64_t fi = 0;
+    if (has) fi = use_retry ? retry_in[si] : si;
+    uint64_t key[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key[j] = has ? front[fi * W + j] : kFlag;
+
+    // ---- pass 1: rows, successor count, enabledness, safety
+    uint64_t labs = 0, cnt = 0;
+    bool anyloc = false, anyen = false;
+    for (int p = 0; p < N; p++) {
+      uint32_t row = kNone;
+      if (has) {
+        const uint32_t sp = kget<W>(key, M[H.off_floc + p]);
+        if (sp < M[H.off_rowcnt + p]) {
+          row = M[H.off_rowbase + p] + sp;
+          if (M[H.off_rowflags + row] & kRowBad) row = kNone;
+        }
+        if (row == kNone) bad = true;
+      }
+      rows[p * 64 + lane] = row;
+      if (row != kNone) {
+        const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
+        cnt += l1 - l0;
+        anyloc |= (M[H.off_rowflags + row] & kRowHasLocal) != 0;
+        labs |= rowlab(row);
+        if (safety && p == N - 1)
+          for (uint32_t t = l0; t < l1; t++)
+            if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) viol = true;
+      }
+    }
+    for (uint64_t L = labs; L; L &= L - 1) {
+      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
+      uint32_t en = 0;
+      for (int p = 0; p < N; p++) {
+        const uint32_t row = rows[p * 64 + lane];
+        if (row != kNone && ((rowlab(row) >> l) & 1ull)) en |= 1u << p;
+      }
+      const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
+      for (uint32_t j = rs; j < re; j++) {
+        const uint32_t r = M[H.off_rules + j];
+        if (r & ~en) continue;
+        anyen = true;
+        uint64_t prod = 1;
+        for (uint32_t rr = r; rr; rr &= rr - 1) {
+          const int q = __ffs(rr) - 1;
+          prod *= M[H.off_syn_tcnt + syn_entry(rows[q * 64 + lane], l)];
+        }
+        cnt += prod;
+        if (safety && ((r >> (N - 1)) & 1u)) {
+          const uint32_t e = syn_entry(rows[(N - 1) * 64 + lane], l);
+          const uint32_t ts = M[H.off_syn_tstart + e], tc = M[H.off_syn_tcnt + e];
+          for (uint32_t i = 0; i < tc; i++)
+            if ((M[H.off_syn_tgt + ts + i] & fmask_last) == 1u) viol = true;
+        }
+      }
+    }
+    if (a.property == 1 && has && !anyloc && !anyen) viol = true;  // cu:1134-1151
+    if (!has) cnt = 0;
+
+    // ---- reserve candidate space (as in k_expand)
+    uint64_t total = 0;
+    const uint64_t off = wave_excl_scan(cnt, &total);
+    uint64_t obase = 0;
+    int ok = 1;
+    if (total) {
+      if (cfull) {
+        ok = 0;
+      } else if (total > cleft) {
+        release(cbase, cleft);
+        const uint64_t left = (nwork - b0 + nw * 64 - 1) / (nw * 64);
+        uint64_t want = total * left;
+        want = want < kChunk ? want : kChunk;
+        want = want > total ? want : total;
+        uint64_t got = 0;
+        if (lane == 0)
+          got = atomicAdd(reinterpret_cast<unsigned long long *>(&C->cand_count),
+                          static_cast<unsigned long long>(want));
+        got = __shfl(got, 0, 64);
+        const uint64_t avail = got < a.cand_cap ? a.cand_cap - got : 0;
+        if (avail >= total) {
+          cbase = got;
+          cleft = want < avail ? want : avail;
+        } else {
+          release(got, want);
+          cbase = 0;
+          cleft = 0;
+          cfull = true;
+          ok = (total > a.cand_cap) ? -1 : 0;
+        }
+      }
+      if (ok > 0) {
+        obase = cbase;
+        cbase += total;
+        cleft -= total;
+      }
+    }
+    if (ok <= 0) {
+      if (ok < 0 && lane == 0) C->overflow = 2;
+      const uint64_t m = ballot(has);
+      uint64_t rb = 0;
+      if (lane == 0) rb = atomicAdd(reinterpret_cast<unsigned long long *>(retry_out_count),
+                                    static_cast<unsigned long long>(__popcll(m)));
+      rb = __shfl(rb, 0, 64);
+      if (has) retry_out[rb + __popcll(m & ((1ull << lane) - 1ull))] = fi;
+      continue;
+    }
+    acc_trans += total;
+    acc_states += __popcll(ballot(has));
+
+    // ---- pass 2: emit
+    uint64_t pos = obase + off;
+    if (has) {
+      for (int p = 0; p < N; p++) {
+        const uint32_t row = rows[p * 64 + lane];
+        if (row == kNone) continue;
+        const uint32_t fl = M[H.off_floc + p];
+        const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
+        for (uint32_t t = l0; t < l1; t++) {
+          uint64_t s[W];
+#pragma unroll
+          for (int j = 0; j < W; j++) s[j] = key[j];
+          kset<W>(s, fl, M[H.off_loc_tgt + t]);
+#pragma unroll
+          for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
+          pos++;
+        }
+      }
+      for (uint64_t L = labs; L; L &= L - 1) {
+        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
+        uint32_t en = 0;
+        for (int p = 0; p < N; p++) {
+          const uint32_t row = rows[p * 64 + lane];
+          if (row != kNone && ((rowlab(row) >> l) & 1ull)) en |= 1u << p;
+        }
+        const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
+        for (uint32_t j = rs; j < re; j++) {
+          const uint32_t r = M[H.off_rules + j];
+          if (r & ~en) continue;
+          uint32_t prod = 1;
+          for (uint32_t rr = r; rr; rr &= rr - 1)
+            prod *= M[H.off_syn_tcnt + syn_entry(rows[(__ffs(rr) - 1) * 64 + lane], l)];
+          for (uint32_t k = 0; k < prod; k++) {
+            uint64_t s[W];
+#pragma unroll
+            for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
+            uint32_t rem = k;
+            for (uint32_t rr = r; rr; rr &= rr - 1) {  // lowest process varies fastest
+              const int q = __ffs(rr) - 1;
+              const uint32_t e = syn_entry(rows[q * 64 + lane], l);
+              const uint32_t tc = M[H.off_syn_tcnt + e];
+              const uint32_t d = rem % tc;
+              rem /= tc;
+              kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + M[H.off_syn_tstart + e] + d]);
+            }
+#pragma unroll
+            for (int jj = 0; jj < W; jj++) a.cand[pos * W + jj] = s[jj];
+            pos++;
+          }
+        }
+      }
+    }
+  }
+  release(cbase, cleft);
+  if (lane == 0) {
+    if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
+                             static_cast<unsigned long long>(acc_trans));
+    if (acc_states) atomicAdd(reinterpret_cast<unsigned long long *>(&C->expanded),
+                              static_cast<unsigned long long>(acc_states));
+  }
+  if (ballot(viol) && lane == 0) C->violation = 1;
+  if (ballot(bad) && lane == 0) C->bad_state = 1;
+}
+
+// ---------------------------------------------------------------------------
 // FIND-OR-PUT over the candidate buffer.  New keys are staged per wave in
 // LDS and appended to the next frontier in blocks of up to kStage keys, one
 // atomicAdd per block: a single contended counter would otherwise serialise
@@ -530,7 +765,16 @@ size_t expand_scratch_bytes(uint32_t threads) {
 template <int W, int WT>
 static hipError_t expand_w(const ExpandArgs &a, int mode, bool lds, uint32_t blocks,
                            uint32_t threads, hipStream_t s) {
-  const size_t shm = (lds ? ((a.hdr.total_words + 3u) & ~3u) * 4u : 0u) + expand_scratch_bytes(threads);
+  const size_t model_b = lds ? ((a.hdr.total_words + 3u) & ~3u) * 4u : 0u;
+  if (mode == kModePlain && a.per_state) {
+    const size_t shm = model_b + static_cast<size_t>(threads / 64) * a.hdr.nprocs * 64 * 4;
+    if (lds)
+      hipLaunchKernelGGL((k_expand_tps<W, true>), dim3(blocks), dim3(threads), shm, s, a);
+    else
+      hipLaunchKernelGGL((k_expand_tps<W, false>), dim3(blocks), dim3(threads), shm, s, a);
+    return hipGetLastError();
+  }
+  const size_t shm = model_b + expand_scratch_bytes(threads);
   if (mode == kModePlain) {
     if (lds)
       hipLaunchKernelGGL((k_expand<W, true, kModePlain, W>), dim3(blocks), dim3(threads), shm, s, a);

@@ -125,6 +125,7 @@ int compile_model(const Gpf &g, Compiled *out) {
   h.nrows = nrows;
 
   std::vector<uint32_t> loc_start, syn_start, rowflags, loc_tgt, syn_lab, syn_tstart, syn_tcnt, syn_tgt;
+  std::vector<uint32_t> rowlab;  // per row: 64-bit label mask as two words
   loc_start.reserve(nrows + 1);
   syn_start.reserve(nrows + 1);
   rowflags.reserve(nrows);
@@ -148,6 +149,8 @@ int compile_model(const Gpf &g, Compiled *out) {
         o2 = bits(tmp, (pos + 1) * g.nbits_offset, g.nbits_offset);
       if (o1 > o2 || o2 > nt) {
         rowflags.push_back(kRowBad);
+        rowlab.push_back(0);
+        rowlab.push_back(0);
         continue;
       }
       uint32_t flags = 0;
@@ -171,7 +174,9 @@ int compile_model(const Gpf &g, Compiled *out) {
             loc_tgt.push_back(v - 1);
         }
       }
+      uint64_t lm = 0;
       for (auto &kv : by_label) {  // ascending labels
+        if (kv.first < 64) lm |= 1ull << kv.first;
         syn_lab.push_back(kv.first);
         syn_tstart.push_back(static_cast<uint32_t>(syn_tgt.size()));
         syn_tcnt.push_back(static_cast<uint32_t>(kv.second.size()));
@@ -180,6 +185,8 @@ int compile_model(const Gpf &g, Compiled *out) {
         any_sync = true;
       }
       rowflags.push_back(flags);
+      rowlab.push_back(static_cast<uint32_t>(lm));
+      rowlab.push_back(static_cast<uint32_t>(lm >> 32));
     }
   }
   loc_start.push_back(static_cast<uint32_t>(loc_tgt.size()));
@@ -188,6 +195,7 @@ int compile_model(const Gpf &g, Compiled *out) {
   // rules per label (cu:997-1009, 1466-1488)
   const uint32_t nlab = any_sync ? max_label + 1 : 0;
   h.nlab = nlab;
+  h.lab64 = nlab <= 64 ? 1u : 0u;
   std::vector<uint32_t> rule_start, rules;
   const uint32_t k2 = 32 / g.nbits_syncbits_offset, per = 32 / N;
   for (uint32_t a = 0; a < nlab; a++) {
@@ -232,6 +240,7 @@ int compile_model(const Gpf &g, Compiled *out) {
   h.off_syn_tgt = put(syn_tgt);
   h.off_rule_start = put(rule_start);
   h.off_rules = put(rules);
+  h.off_rowlab = put(rowlab);
   h.total_words = static_cast<uint32_t>(B.size());
   out->hdr = h;
   return GX_OK;

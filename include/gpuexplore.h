@@ -76,7 +76,7 @@ typedef struct gx_options {
   uint32_t blocks;           /* -b: workgroups per launch (0 = auto) */
   uint32_t threads;          /* -t: threads per workgroup, multiple of 64 (0 = auto) */
   uint32_t kernel_iters;     /* -k: BFS levels enqueued per host sync (0 = 1) */
-  uint32_t bucket_bytes;     /* hash bucket width: 64, 128 or 256 (0 = 128) */
+  uint32_t bucket_bytes;     /* hash bucket width: 64, 128 or 256 (0 = 64 for slots of <= 2 words, else 128) */
   uint64_t q_ints;           /* -q: hash table size in 32-bit ints (0 = auto) */
   int32_t property;          /* GX_PROP_* */
   int32_t por;               /* --por */
@@ -86,6 +86,8 @@ typedef struct gx_options {
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
   uint64_t max_states;       /* stop after the level that reaches this many states (0 = none) */
+  uint32_t expand_kernel;    /* successor kernel: 0 auto, 1 process groups, 2 one state per lane */
+  uint32_t reserved;
 } gx_options;
 
 void gx_options_default(gx_options *o);

@@ -79,6 +79,19 @@ def test_small_models_all_modes(gpu, oracle_mod, model_dir, name, net):
             assert same(a, b), (prop, por, prov, a, b)
 
 
+@pytest.mark.parametrize("name,net", small_models(), ids=[n for n, _ in small_models()])
+def test_small_models_group_kernel(gpu, oracle_mod, model_dir, name, net):
+    """plain mode through the process-group kernel (the default picks the
+    one-state-per-lane kernel for these models)"""
+    p = os.path.join(model_dir, name)
+    net.write(p)
+    for prop in (0, 1, 2):
+        say(name, prop, "group kernel")
+        a = oracle_mod.explore(p, prop)
+        b = gpu_run(gpu, p, prop, expand_kernel=1)
+        assert same(a, b), (prop, a, b)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_config1(gpu, oracle_mod, model_dir, seed):
     p = os.path.join(model_dir, "c1")
