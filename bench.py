@@ -89,6 +89,18 @@ def main() -> int:
     algo_bytes = BYTES_PER_FINDORPUT * (lookups + inserts)
     achieved = algo_bytes / t_hash / 1e9
 
+    # HBM traffic per k_insert launch from the committed rocprofv3 --pmc runs of
+    # this same command (FETCH_SIZE and WRITE_SIZE, scaled by the counters'
+    # reading of a streaming copy of known size), when the workload matches
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "k_insert_config2_traffic.json")
+    if os.path.exists(prof) and args.q_log2 == 34 and args.seed == 1 and not args.bucket_bytes \
+            and not args.expand_kernel:
+        pj = json.load(open(prof))
+        for k, v in pj["kernels"].items():
+            if "k_insert" in k:
+                traffic = v["bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
@@ -133,7 +145,8 @@ def main() -> int:
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/k_insert_config2_traffic.json" if traffic else None,
             "kernel": "k_insert (FIND-OR-PUT)",
             "algorithmic_bytes_per_launch": algo_bytes / launches,
             "avg_launch_us": 1e6 * t_hash / launches,
