@@ -30,6 +30,53 @@ PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E, published
 BYTES_PER_FINDORPUT = 128  # one 128-byte bucket line per lookup, one per insert
 
 
+def batched_vs_plain(gx, gen, tmp: str) -> dict:
+    """64 models of uneven size (gen.config4_models, seeds 1000-1063; half
+    -d, half -p): one gx_explore_batch with autotuned settings against one
+    exploration per model (explorers created before timing)."""
+    from gpuexplore_amd import autotune
+
+    paths, props = [], []
+    for i, (net, flag) in enumerate(gen.config4_models(1000, 64)):
+        p = os.path.join(tmp, f"c4_{i}")
+        net.write(p)
+        paths.append(p)
+        props.append(gx.PROP_DEADLOCK if flag == "-d" else gx.PROP_SAFETY)
+    models = [gx.Model.load(p) for p in paths]
+    q_batch = 64 << 22
+    tuned = autotune.tune_batch(models, props, q_batch, repeats=2)
+    best = tuned["best"]
+    runs = []
+    for _ in range(3):
+        res, stats, rc = gx.explore_batch(models, props, q_ints=q_batch, **best)
+        runs.append(stats)
+    t_batch = min(s["t_explore_s"] for s in runs)
+    states = runs[0]["states"]
+    explorers = [gx.Explorer(m, property=pr, q_ints=1 << 22) for m, pr in zip(models, props)]
+    plain_t = []
+    for _ in range(2):
+        t = 0.0
+        for i, ex in enumerate(explorers):
+            r = ex.run()
+            if (r["states"], r["transitions"]) != (res[i]["states"], res[i]["transitions"]):
+                raise RuntimeError(f"batched and plain counts differ for model {i}")
+            t += r["t_explore_s"]
+        plain_t.append(t)
+    t_plain = min(plain_t)
+    return {
+        "workload": "config4: 64 models, N 4..14, 4..32 states, 32 -d + 32 -p",
+        "states": states,
+        "batched_new_states_per_s": states / t_batch,
+        "plain_new_states_per_s": states / t_plain,
+        "ratio": t_plain / t_batch,
+        "t_batched_s": t_batch,
+        "t_plain_s": t_plain,
+        "autotuned": best,
+        "rounds": runs[0]["rounds"],
+        "graph_launches": runs[0]["graph_launches"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +88,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -101,6 +149,8 @@ def main() -> int:
             if "k_insert" in k:
                 traffic = v["bytes_per_launch"]
 
+    batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
@@ -159,6 +209,7 @@ def main() -> int:
             "t_expand_s_per_step": sum(r["t_expand_s"] for r in runs) / args.steps,
         },
         "cpu_baseline": cpu,
+        "batched_vs_plain": batch,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -73,6 +73,32 @@ class Result(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("threads", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32),
+        ("levels_per_sync", ctypes.c_uint32),
+        ("bucket_bytes", ctypes.c_uint32),
+        ("q_ints", ctypes.c_uint64),
+        ("hash_seed", ctypes.c_uint64),
+        ("cand_capacity", ctypes.c_uint64),
+        ("use_graph", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("t_explore_s", ctypes.c_double),
+        ("t_init_s", ctypes.c_double),
+        ("rounds", ctypes.c_uint64),
+        ("graph_launches", ctypes.c_uint64),
+        ("lookups", ctypes.c_uint64),
+        ("states", ctypes.c_uint64),
+        ("transitions", ctypes.c_uint64),
+    ]
+
+
 class ModelInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "nr_procs", "bits_act", "proc_nrstates", "bits_statevector", "sv_nints", "liveness",
@@ -92,6 +118,11 @@ SIGNATURES = {
     "gx_explorer_create": (ctypes.c_int, [_P, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
     "gx_explorer_run": (ctypes.c_int, [_P, ctypes.POINTER(Result)]),
     "gx_explorer_free": (None, [_P]),
+    "gx_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
+    "gx_explore_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_uint64), ctypes.c_int,
+                                        ctypes.POINTER(BatchOptions), ctypes.POINTER(Result),
+                                        ctypes.POINTER(BatchStats)]),
     "gx_table_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                        ctypes.POINTER(_P)]),
     "gx_table_free": (None, [_P]),
@@ -210,6 +241,27 @@ class Explorer:
         if getattr(self, "h", None) and self.h.value and _lib is not None:
             _lib.gx_explorer_free(self.h)
             self.h = None
+
+
+def explore_batch(models, properties=None, weights=None, **kw):
+    """gx_explore_batch.  Returns (list of per-instance result dicts, stats
+    dict, status)."""
+    n = len(models)
+    o = BatchOptions()
+    lib().gx_batch_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    hs = (ctypes.c_void_p * n)(*[m.h.value for m in models])
+    props = (ctypes.c_int32 * n)(*(properties or [0] * n))
+    w = (ctypes.c_uint64 * n)(*weights) if weights is not None else None
+    res = (Result * n)()
+    st = BatchStats()
+    rc = lib().gx_explore_batch(ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p)), props,
+                                ctypes.cast(w, ctypes.POINTER(ctypes.c_uint64)) if w is not None else None,
+                                n, ctypes.byref(o), res, ctypes.byref(st))
+    if rc not in (GX_OK, -10):
+        raise GxError(rc, "explore_batch")
+    return [r.as_dict() for r in res], {k: getattr(st, k) for k, _ in st._fields_}, rc
 
 
 class Table:

@@ -1,0 +1,534 @@
+// gx_batch.hip -- many independent models explored together.
+//
+// Every instance owns a partition of one visited set and two frontier
+// regions; the instances advance one BFS level per round, in lockstep:
+//   k_batch_expand   64-state chunks of one instance per wave (the model is
+//                    wave-uniform), state-per-lane successor generation
+//   k_batch_insert   FIND-OR-PUT into the candidate's instance partition
+//   k_batch_advance  per-instance level bookkeeping and termination, then
+//                    the chunk prefix of the next round (one workgroup)
+// A round reads all its sizes from device memory, so `levels_per_sync`
+// rounds are captured once in a hipGraph and replayed until every instance
+// has stopped.  Semantics per instance are those of gx_explore (plain
+// method with an optional deadlock or safety check).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "../../include/gpuexplore.h"
+#include "gx_kernels.h"
+#include "gx_model.h"
+#include "gx_tps.h"
+
+namespace gx {
+
+struct BatchInst {
+  ModelHdr hdr;
+  uint32_t model_off;   // into the concatenated model buffer (words)
+  int32_t property;
+  uint64_t tb, nb;      // table partition: first bucket, buckets
+  uint64_t fo, fcap;    // frontier region: first entry, capacity
+  uint64_t front_count[2];
+  uint64_t inserts, transitions, expanded;
+  uint64_t chunk_prefix;  // first chunk of this instance in the round
+  uint32_t cur, level;
+  uint32_t violation, table_full, overflow, bad_state, spin_timeout, stop;
+};
+
+struct BatchCtrl {
+  uint64_t cand_count;
+  uint64_t lookups;
+  uint64_t total_chunks;
+  uint64_t deferred_count[2];
+  uint32_t deferred_cur;
+  uint32_t use_deferred;
+  uint32_t active;       // instances not stopped
+  uint32_t rounds;
+  uint32_t n;
+  uint32_t cand_overflow;
+};
+
+struct BatchArgs {
+  BatchInst *inst;
+  BatchCtrl *ctrl;
+  const uint32_t *models;
+  uint64_t *front[2];   // pools, W words per entry
+  uint64_t *cand;
+  uint32_t *cand_inst;
+  uint64_t cand_cap;
+  uint64_t *deferred[2];  // chunk descriptors: instance << 40 | first state
+  uint64_t *table;
+  uint64_t seed;
+  uint32_t max_procs;
+};
+
+__device__ __forceinline__ uint32_t find_instance(const BatchInst *I, uint32_t n, uint64_t c) {
+  uint32_t lo = 0, hi = n;  // last instance with chunk_prefix <= c
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (I[mid].chunk_prefix <= c)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int W>
+__global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
+  extern __shared__ uint32_t smem[];
+  BatchCtrl *C = a.ctrl;
+  if (C->active == 0) return;
+  uint32_t *rows = smem + static_cast<uint32_t>(threadIdx.x >> 6) * (a.max_procs * 64u);
+  const uint32_t n = C->n;
+  const uint32_t use_def = C->use_deferred;
+  const uint32_t dc = C->deferred_cur;
+  const uint64_t nchunks = use_def ? C->deferred_count[dc] : C->total_chunks;
+  const uint64_t *def_in = a.deferred[dc];
+  uint64_t *def_out = a.deferred[dc ^ 1u];
+  uint64_t *def_out_count = &C->deferred_count[dc ^ 1u];
+  const int lane = lane_id();
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
+  for (uint64_t c = gw; c < nchunks; c += nw) {
+    uint32_t i;
+    uint64_t start;
+    if (use_def) {
+      const uint64_t d = def_in[c];
+      i = static_cast<uint32_t>(d >> 40);
+      start = d & ((1ull << 40) - 1);
+    } else {
+      i = find_instance(a.inst, n, c);
+      start = (c - a.inst[i].chunk_prefix) * 64;
+    }
+    BatchInst *I = &a.inst[i];
+    const ModelHdr H = I->hdr;
+    const uint32_t *M = a.models + I->model_off;
+    const uint32_t cur = I->cur;
+    const uint64_t cnt_front = I->front_count[cur];
+    const uint64_t si = start + lane;
+    const bool has = si < cnt_front;
+    uint64_t key[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key[j] = has ? a.front[cur][(I->fo + si) * W + j] : kFlag;
+    const TpsInfo inf = tps_scan<W>(M, H, key, has, rows, lane, I->property == 2);
+    const bool viol = inf.viol || (I->property == 1 && inf.dead);
+    if (ballot(viol) && lane == 0) I->violation = 1;
+    if (ballot(inf.bad) && lane == 0) I->bad_state = 1;
+    uint64_t total = 0;
+    const uint64_t off = wave_excl_scan(inf.cnt, &total);
+    uint64_t obase = 0;
+    const int ok = ch.reserve<W>(total, (nchunks - c + nw - 1) / nw, &obase);
+    if (ok <= 0) {
+      if (lane == 0) {
+        if (ok < 0) C->cand_overflow = 1;
+        const uint64_t at = atomicAdd(reinterpret_cast<unsigned long long *>(def_out_count), 1ull);
+        def_out[at] = (static_cast<uint64_t>(i) << 40) | start;
+      }
+      continue;
+    }
+    if (lane == 0 && total) {
+      atomicAdd(reinterpret_cast<unsigned long long *>(&I->transitions), static_cast<unsigned long long>(total));
+      atomicAdd(reinterpret_cast<unsigned long long *>(&I->expanded),
+                static_cast<unsigned long long>(__popcll(ballot(has))));
+    }
+    if (has) {
+      uint64_t pos = obase + off;
+      tps_emit<W>(M, H, key, rows, lane, inf.labs, [&](const uint64_t (&sv)[W]) {
+#pragma unroll
+        for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+        a.cand_inst[pos] = i;
+        pos++;
+      });
+    }
+  }
+  ch.release_all<W>();
+}
+
+template <int W, int BW>
+__global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
+  BatchCtrl *C = a.ctrl;
+  if (C->active == 0) return;
+  const uint64_t n = C->cand_count < a.cand_cap ? C->cand_count : a.cand_cap;
+  constexpr int LPB = BW / 2;
+  constexpr int CPW = 64 / LPB;
+  SlotFmt<W> fmt;
+  make_fmt<W, false>(fmt, 0, 0);
+  const int lane = lane_id();
+  const int sub = lane % LPB;
+  const int cg = lane / LPB;
+  const int gbase = cg * LPB;
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint64_t acc_look = 0;
+  for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
+    const uint64_t ci = c0 + cg;
+    bool active = ci < n;
+    uint64_t key[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
+    active = active && key[0] != 0;
+    const uint32_t i = active ? a.cand_inst[ci] : 0u;
+    BatchInst *I = &a.inst[i];
+    Table T{a.table + (active ? I->tb : 0) * BW, active ? I->nb : 1, a.seed};
+    const int st = find_or_put_group<W, BW>(T, key, fmt, key_hash<W>(key, a.seed), sub, gbase, active);
+    acc_look += __popcll(ballot(active && sub == 0));
+    if (active && sub == 0) {
+      if (st == kInserted) {
+        const uint32_t nxt = I->cur ^ 1u;
+        const uint64_t idx =
+            atomicAdd(reinterpret_cast<unsigned long long *>(&I->front_count[nxt]), 1ull);
+        if (idx < I->fcap) {
+#pragma unroll
+          for (int j = 0; j < W; j++) a.front[nxt][(I->fo + idx) * W + j] = key[j];
+        } else {
+          I->overflow = 1;
+        }
+      } else if (st == kFull) {
+        I->table_full = 1;
+      } else if (st == kTimeout) {
+        I->spin_timeout = 1;
+      }
+    }
+  }
+  if (lane == 0 && acc_look)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(acc_look));
+}
+
+// One workgroup of 1024 threads; instances are handled in tiles of 1024.
+__global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
+  __shared__ uint64_t scan[1024];
+  __shared__ uint32_t nact;
+  BatchCtrl *C = a.ctrl;
+  if (C->active == 0) return;
+  const uint32_t tid = threadIdx.x;
+  const bool deferred = C->deferred_count[C->deferred_cur ^ 1u] > 0;
+  __syncthreads();
+  if (tid == 0) {
+    C->cand_count = 0;
+    C->rounds += 1;
+  }
+  if (deferred) {  // the level is not complete: run the deferred chunks
+    if (tid == 0) {
+      const uint32_t dc = C->deferred_cur;
+      C->deferred_count[dc] = 0;
+      C->deferred_cur = dc ^ 1u;
+      C->use_deferred = 1;
+    }
+    return;
+  }
+  if (tid == 0) {
+    C->deferred_count[C->deferred_cur] = 0;
+    C->use_deferred = 0;
+    nact = 0;
+  }
+  __syncthreads();
+  const uint32_t n = C->n;
+  uint64_t carry = 0;
+  for (uint32_t t0 = 0; t0 < n; t0 += 1024) {
+    const uint32_t i = t0 + tid;
+    uint64_t chunks = 0;
+    if (i < n) {
+      BatchInst *I = &a.inst[i];
+      if (!I->stop) {
+        const uint32_t cur = I->cur, nxt = cur ^ 1u;
+        const uint64_t fresh = I->front_count[nxt] < I->fcap ? I->front_count[nxt] : I->fcap;
+        I->inserts += fresh;
+        I->front_count[cur] = 0;
+        I->cur = nxt;
+        I->level += 1u;
+        if (I->violation || I->table_full || I->overflow || I->bad_state || I->spin_timeout) I->stop = 1;
+        if (I->front_count[nxt] == 0) I->stop = 1;
+        if (!I->stop) chunks = (I->front_count[nxt] + 63) / 64;
+      }
+      if (!I->stop) atomicAdd(&nact, 1u);
+    }
+    scan[tid] = chunks;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive scan
+      const uint64_t v = tid >= d ? scan[tid - d] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    if (i < n) a.inst[i].chunk_prefix = carry + scan[tid] - chunks;
+    carry += scan[1023];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    C->total_chunks = carry;
+    C->active = nact;
+  }
+}
+
+template <int W>
+static hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  const size_t shm = static_cast<size_t>(threads / 64) * a.max_procs * 64 * 4;
+  hipLaunchKernelGGL((k_batch_expand<W>), dim3(blocks), dim3(threads), shm, s, a);
+  switch (BW) {
+    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(blocks), dim3(threads), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
+  return hipGetLastError();
+}
+
+static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStream_t s) {
+  // the initial states: one candidate per instance
+#define GX_BI(Wk)                                                                               \
+  case Wk:                                                                                      \
+    switch (BW) {                                                                               \
+      case 8: hipLaunchKernelGGL((k_batch_insert<Wk, 8>), dim3(64), dim3(256), 0, s, a); break;   \
+      case 16: hipLaunchKernelGGL((k_batch_insert<Wk, 16>), dim3(64), dim3(256), 0, s, a); break; \
+      case 32: hipLaunchKernelGGL((k_batch_insert<Wk, 32>), dim3(64), dim3(256), 0, s, a); break; \
+      default: return hipErrorInvalidValue;                                                     \
+    }                                                                                           \
+    break;
+  switch (W) {
+    GX_BI(1)
+    GX_BI(2)
+    GX_BI(3)
+    GX_BI(4)
+    GX_BI(5)
+    default: return hipErrorInvalidValue;
+  }
+#undef GX_BI
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
+  return hipGetLastError();
+}
+
+static hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
+                                 hipStream_t s) {
+  switch (W) {
+    case 1: return launch_round<1>(a, BW, blocks, threads, s);
+    case 2: return launch_round<2>(a, BW, blocks, threads, s);
+    case 3: return launch_round<3>(a, BW, blocks, threads, s);
+    case 4: return launch_round<4>(a, BW, blocks, threads, s);
+    case 5: return launch_round<5>(a, BW, blocks, threads, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+namespace {
+
+struct Buf {
+  void *p = nullptr;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) { return hipMalloc(&p, b ? b : 8); }
+};
+
+int map_err(hipError_t e) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GX_E_NOMEM;
+  return GX_E_HIP;
+}
+
+#define BX_TRY(expr)                          \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return map_err(e_); \
+  } while (0)
+
+}  // namespace
+
+extern "C" void gx_batch_options_default(gx_batch_options *o) {
+  memset(o, 0, sizeof *o);
+  o->threads = 256;
+  o->levels_per_sync = 8;
+  o->use_graph = 1;
+  o->hash_seed = 0x5eed5eed5eedull;
+}
+
+extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *properties,
+                                const uint64_t *weights, int n, const gx_batch_options *o,
+                                gx_result *results, gx_batch_stats *stats) {
+  if (!models || !o || !results || n <= 0 || n > (1 << 20)) return GX_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return GX_E_NODEVICE;
+  memset(results, 0, sizeof(gx_result) * n);
+  if (stats) memset(stats, 0, sizeof *stats);
+  // layout: common key width, concatenated models
+  int W = 1;
+  uint32_t max_procs = 1;
+  std::vector<uint32_t> mbuf;
+  std::vector<BatchInst> inst(n);
+  for (int i = 0; i < n; i++) {
+    const gx_model *m = models[i];
+    if (!m || m->por) return GX_E_ARG;
+    const ModelHdr &H = m->comp.hdr;
+    if (!H.lab64) return GX_E_LIMIT;
+    W = std::max<int>(W, static_cast<int>(H.key_words));
+    max_procs = std::max(max_procs, H.nprocs);
+    memset(&inst[i], 0, sizeof(BatchInst));
+    inst[i].hdr = H;
+    inst[i].model_off = static_cast<uint32_t>(mbuf.size());
+    inst[i].property = properties ? properties[i] : GX_PROP_NONE;
+    inst[i].cur = 1;
+    inst[i].level = 0xFFFFFFFFu;
+    mbuf.insert(mbuf.end(), m->comp.words.begin(), m->comp.words.end());
+    while (mbuf.size() % 4) mbuf.push_back(0);
+  }
+  if (W > kMaxKeyWords) return GX_E_LIMIT;
+  const uint32_t bb = o->bucket_bytes ? o->bucket_bytes : (W <= 2 ? 64u : 128u);
+  if (bb != 64 && bb != 128 && bb != 256) return GX_E_ARG;
+  const int BW = static_cast<int>(bb / 8);
+  if (BW < W) return GX_E_ARG;
+  const uint64_t spb = static_cast<uint64_t>(BW / W);
+  uint32_t threads = o->threads ? o->threads : 256;
+  threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (threads + 63) / 64 * 64));
+  if (static_cast<size_t>(threads / 64) * max_procs * 256 > 64 * 1024) return GX_E_LIMIT;
+  int dev = 0;
+  BX_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  BX_TRY(hipGetDeviceProperties(&prop, dev));
+  const uint32_t blocks = o->blocks ? o->blocks : prop.multiProcessorCount * (2048 / threads);
+  // table partitions proportional to the weights (equal without weights)
+  // default: 2^22 ints (16 MiB) per instance
+  const uint64_t q_ints = o->q_ints ? o->q_ints : (static_cast<uint64_t>(n) << 22);
+  const uint64_t total_buckets = std::max<uint64_t>(q_ints * 4 / bb, static_cast<uint64_t>(n));
+  double wsum = 0;
+  for (int i = 0; i < n; i++) wsum += weights ? std::max<double>(1.0, static_cast<double>(weights[i])) : 1.0;
+  uint64_t tb = 0, fo = 0;
+  for (int i = 0; i < n; i++) {
+    const double wi = weights ? std::max<double>(1.0, static_cast<double>(weights[i])) : 1.0;
+    uint64_t nb = static_cast<uint64_t>(static_cast<double>(total_buckets) * wi / wsum);
+    nb = std::max<uint64_t>(nb, 1);
+    if (tb + nb > total_buckets) nb = total_buckets > tb ? total_buckets - tb : 0;
+    if (nb == 0) return GX_E_ARG;
+    inst[i].tb = tb;
+    inst[i].nb = nb;
+    inst[i].fo = fo;
+    inst[i].fcap = nb * spb;  // states fit the partition's slots
+    tb += nb;
+    fo += inst[i].fcap;
+  }
+  const uint64_t front_total = fo;
+  uint64_t cand_cap = o->cand_capacity ? o->cand_capacity : std::max<uint64_t>(1u << 22, front_total);
+  cand_cap = std::min<uint64_t>(cand_cap, 0xFFFFFFFFull);
+  const uint64_t max_chunks = front_total / 64 + static_cast<uint64_t>(n) + 1;
+
+  const auto t_init0 = std::chrono::steady_clock::now();
+  Buf d_inst, d_ctrl, d_models, d_f0, d_f1, d_cand, d_cinst, d_def0, d_def1, d_table;
+  BX_TRY(d_inst.alloc(sizeof(BatchInst) * n));
+  BX_TRY(d_ctrl.alloc(sizeof(BatchCtrl)));
+  BX_TRY(d_models.alloc(mbuf.size() * 4));
+  BX_TRY(d_f0.alloc(front_total * W * 8));
+  BX_TRY(d_f1.alloc(front_total * W * 8));
+  BX_TRY(d_cand.alloc(cand_cap * W * 8));
+  BX_TRY(d_cinst.alloc(cand_cap * 4));
+  BX_TRY(d_def0.alloc(max_chunks * 8));
+  BX_TRY(d_def1.alloc(max_chunks * 8));
+  BX_TRY(d_table.alloc(total_buckets * bb));
+  hipStream_t st;
+  BX_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::unique_ptr<void, void (*)(void *)> sg(st, [](void *s) { (void)hipStreamDestroy(static_cast<hipStream_t>(s)); });
+  BX_TRY(hipMemsetAsync(d_table.p, 0, total_buckets * bb, st));
+  BX_TRY(hipMemcpyAsync(d_models.p, mbuf.data(), mbuf.size() * 4, hipMemcpyHostToDevice, st));
+  BX_TRY(hipMemcpyAsync(d_inst.p, inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice, st));
+  BatchCtrl c0{};
+  c0.n = static_cast<uint32_t>(n);
+  c0.active = static_cast<uint32_t>(n);
+  c0.cand_count = static_cast<uint64_t>(n);
+  BX_TRY(hipMemcpyAsync(d_ctrl.p, &c0, sizeof c0, hipMemcpyHostToDevice, st));
+  // initial states: one all-zero key per instance (cu:668-682)
+  std::vector<uint64_t> s0(static_cast<size_t>(n) * W, kFlag);
+  std::vector<uint32_t> s0i(n);
+  for (int i = 0; i < n; i++) s0i[i] = static_cast<uint32_t>(i);
+  BX_TRY(hipMemcpyAsync(d_cand.p, s0.data(), s0.size() * 8, hipMemcpyHostToDevice, st));
+  BX_TRY(hipMemcpyAsync(d_cinst.p, s0i.data(), s0i.size() * 4, hipMemcpyHostToDevice, st));
+  BatchArgs a{};
+  a.inst = static_cast<BatchInst *>(d_inst.p);
+  a.ctrl = static_cast<BatchCtrl *>(d_ctrl.p);
+  a.models = static_cast<const uint32_t *>(d_models.p);
+  a.front[0] = static_cast<uint64_t *>(d_f0.p);
+  a.front[1] = static_cast<uint64_t *>(d_f1.p);
+  a.cand = static_cast<uint64_t *>(d_cand.p);
+  a.cand_inst = static_cast<uint32_t *>(d_cinst.p);
+  a.cand_cap = cand_cap;
+  a.deferred[0] = static_cast<uint64_t *>(d_def0.p);
+  a.deferred[1] = static_cast<uint64_t *>(d_def1.p);
+  a.table = static_cast<uint64_t *>(d_table.p);
+  a.seed = o->hash_seed;
+  a.max_procs = max_procs;
+  BX_TRY(launch_insert_only(a, W, BW, st));
+  BX_TRY(hipStreamSynchronize(st));
+  const double t_init = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_init0).count();
+
+  const uint32_t k = std::max<uint32_t>(1, o->levels_per_sync);
+  BatchCtrl ch{};
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (o->use_graph) {
+    BX_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    for (uint32_t r = 0; r < k && ce == hipSuccess; r++) ce = launch_round_w(a, W, BW, blocks, threads, st);
+    hipError_t ee = hipStreamEndCapture(st, &graph);
+    BX_TRY(ce);
+    BX_TRY(ee);
+    BX_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gg{graph, exec};
+  uint64_t launches = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    if (exec) {
+      BX_TRY(hipGraphLaunch(exec, st));
+    } else {
+      for (uint32_t r = 0; r < k; r++) BX_TRY(launch_round_w(a, W, BW, blocks, threads, st));
+    }
+    launches++;
+    BX_TRY(hipMemcpyAsync(&ch, d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
+    BX_TRY(hipStreamSynchronize(st));
+    if (ch.active == 0) break;
+  }
+  const double t_explore = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  BX_TRY(hipMemcpy(inst.data(), d_inst.p, sizeof(BatchInst) * n, hipMemcpyDeviceToHost));
+  int rc = GX_OK;
+  for (int i = 0; i < n; i++) {
+    const BatchInst &I = inst[i];
+    gx_result &r = results[i];
+    r.states = I.inserts;
+    r.transitions = I.transitions;
+    r.levels = I.level;
+    r.inserts = I.inserts;
+    r.violation = I.violation ? 1 : 0;
+    r.table_full = I.table_full ? 1 : 0;
+    r.table_bytes = I.nb * bb;
+    r.t_explore_s = t_explore;
+    r.t_init_s = t_init;
+    if (I.bad_state) rc = GX_E_STATE;
+    if ((I.overflow || I.spin_timeout) && rc == GX_OK) rc = GX_E_CAPACITY;
+  }
+  if (ch.cand_overflow && rc == GX_OK) rc = GX_E_CAPACITY;
+  if (stats) {
+    stats->t_explore_s = t_explore;
+    stats->t_init_s = t_init;
+    stats->rounds = ch.rounds;
+    stats->graph_launches = launches;
+    stats->lookups = ch.lookups;
+    for (int i = 0; i < n; i++) {
+      stats->states += inst[i].inserts;
+      stats->transitions += inst[i].transitions;
+    }
+  }
+  return rc;
+}

@@ -40,6 +40,65 @@ struct InsertArgs {
   int append;       // append new keys to front[cur ^ 1]
 };
 
+constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic
+
+// A wave's current chunk of the candidate buffer.  Space is reserved with one
+// atomicAdd per chunk, sized for the wave's remaining batches (at most kChunk
+// entries unless one batch needs more); unused chunk tails are released by
+// zeroing their first words (a zero first word is never a key: keys carry
+// bit 63), which the FIND-OR-PUT kernel skips.  All members are wave-uniform.
+struct CandChunk {
+  uint64_t *cand;
+  uint64_t cap;
+  uint64_t *count;
+  uint64_t base = 0, left = 0;
+  bool full = false;
+  __device__ CandChunk(uint64_t *c, uint64_t cp, uint64_t *cnt) : cand(c), cap(cp), count(cnt) {}
+  template <int W>
+  __device__ void release(uint64_t b, uint64_t n) {
+    const uint64_t end = (b + n < cap) ? b + n : cap;
+    for (uint64_t i = b + lane_id(); i < end; i += 64) cand[i * W] = 0;
+  }
+  template <int W>
+  __device__ void release_all() {
+    release<W>(base, left);
+    left = 0;
+  }
+  // Reserves `total` entries for this batch (`batches` batches left for the
+  // wave).  Returns 1 and *at on success, 0 if the buffer is full (defer the
+  // batch), -1 if one batch can never fit.
+  template <int W>
+  __device__ int reserve(uint64_t total, uint64_t batches, uint64_t *at) {
+    if (!total) return 1;
+    if (full) return 0;
+    if (total > left) {
+      release<W>(base, left);
+      uint64_t want = total * batches;
+      want = want < kChunk ? want : kChunk;
+      want = want > total ? want : total;
+      uint64_t got = 0;
+      if (lane_id() == 0)
+        got = atomicAdd(reinterpret_cast<unsigned long long *>(count), static_cast<unsigned long long>(want));
+      got = __shfl(got, 0, 64);
+      const uint64_t avail = got < cap ? cap - got : 0;
+      if (avail >= total) {
+        base = got;
+        left = want < avail ? want : avail;
+      } else {
+        release<W>(got, want);
+        base = 0;
+        left = 0;
+        full = true;
+        return total > cap ? -1 : 0;
+      }
+    }
+    *at = base;
+    base += total;
+    left -= total;
+    return 1;
+  }
+};
+
 // Launch one successor-generation pass (dispatches on key width, mode and
 // whether the compiled model fits in LDS).  Returns a hipError_t.
 hipError_t launch_expand(const ExpandArgs &a, int mode, int key_words, int slot_words, bool lds,

@@ -10,11 +10,11 @@
 // All sizes are read from the device control block, so levels can be
 // enqueued back to back without host round trips.
 #include "gx_kernels.h"
+#include "gx_tps.h"
 
 namespace gx {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic (expand)
 constexpr int kStage = 128;        // new keys staged per wave before a flush (insert)
 
 // ---------------------------------------------------------------------------
@@ -87,13 +87,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint64_t acc_trans = 0, acc_states = 0;
   bool viol = false, bad = false;
-  // this wave's current candidate chunk
-  uint64_t cbase = 0, cleft = 0;
-  bool cfull = false;
-  auto release = [&](uint64_t b, uint64_t n) {
-    const uint64_t end = (b + n < a.cand_cap) ? b + n : a.cand_cap;
-    for (uint64_t i = b + lane; i < end; i += 64) a.cand[i * W] = 0;
-  };
+  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
 
   for (uint64_t b0 = gw * G; b0 < nwork; b0 += nw * G) {
     const uint64_t si = b0 + g;
@@ -282,44 +276,8 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
     uint64_t total = 0;
     const uint64_t off = wave_excl_scan(cnt, &total);
     uint64_t obase = 0;
-    int ok = 1;
-    if (total) {
-      // Candidate space is reserved in chunks: one atomicAdd per chunk, sized
-      // for the wave's remaining batches (at most kChunk entries unless one
-      // batch needs more).  Unused chunk tails are released by zero-filling
-      // their first words (a zero first word is never a key: keys carry bit
-      // 63).  A chunk that overruns the buffer defers the wave's batches.
-      if (cfull) {
-        ok = 0;
-      } else if (total > cleft) {
-        release(cbase, cleft);
-        const uint64_t left = (nwork - b0 + nw * G - 1) / (nw * G);
-        uint64_t want = total * left;
-        want = want < kChunk ? want : kChunk;
-        want = want > total ? want : total;
-        uint64_t got = 0;
-        if (lane == 0)
-          got = atomicAdd(reinterpret_cast<unsigned long long *>(&C->cand_count),
-                          static_cast<unsigned long long>(want));
-        got = __shfl(got, 0, 64);
-        const uint64_t avail = got < a.cand_cap ? a.cand_cap - got : 0;
-        if (avail >= total) {
-          cbase = got;
-          cleft = want < avail ? want : avail;
-        } else {
-          release(got, want);
-          cbase = 0;
-          cleft = 0;
-          cfull = true;
-          ok = (total > a.cand_cap) ? -1 : 0;
-        }
-      }
-      if (ok > 0) {
-        obase = cbase;
-        cbase += total;
-        cleft -= total;
-      }
-    }
+    const uint64_t left = (nwork - b0 + nw * G - 1) / (nw * G);
+    const int ok = ch.reserve<W>(total, left, &obase);
     if (ok <= 0) {
       // defer this wave's states to the retry list
       if (ok < 0 && lane == 0) C->overflow = 2;
@@ -371,7 +329,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
            }
          });
   }
-  release(cbase, cleft);
+  ch.release_all<W>();
   if (lane == 0) {
     if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
                              static_cast<unsigned long long>(acc_trans));
@@ -388,11 +346,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
 // For models whose per-process work is small (a few entries per local
 // state) a lane expanding a whole state is cheaper than a group of N lanes:
 // 64 states per wave instead of floor(64/N), and no cross-lane traffic
-// except the wave's output reservation.  The per-process row of the state is
-// kept in LDS ([process][lane], conflict-free); the synchronising labels of a
-// row are a 64-bit mask, so En(a) is built per label and the target list of
-// process q for label a is entry popcount(mask_q & below(a)) of q's row.
-// Semantics are those of the group kernel above.
+// except the wave's output reservation (gx_tps.h has the per-state code).
 template <int W, bool LDS>
 __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   extern __shared__ uint32_t smem[];
@@ -404,8 +358,7 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   const uint32_t *M = LDS ? smem : a.model;
   const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
   const int N = static_cast<int>(H.nprocs);
-  const int wib = threadIdx.x >> 6;
-  uint32_t *rows = smem + xoff + static_cast<uint32_t>(wib) * (N * 64u);
+  uint32_t *rows = smem + xoff + static_cast<uint32_t>(threadIdx.x >> 6) * (N * 64u);
   __syncthreads();
   if (C->stop) return;
   const uint32_t cur = C->cur;
@@ -418,28 +371,11 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   const uint64_t *front = a.front[cur];
   const int lane = lane_id();
   const bool safety = a.property == 2;
-  const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
-  const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
-
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint64_t acc_trans = 0, acc_states = 0;
   bool viol = false, bad = false;
-  uint64_t cbase = 0, cleft = 0;
-  bool cfull = false;
-  auto release = [&](uint64_t b, uint64_t n) {
-    const uint64_t end = (b + n < a.cand_cap) ? b + n : a.cand_cap;
-    for (uint64_t i = b + lane; i < end; i += 64) a.cand[i * W] = 0;
-  };
-  auto rowlab = [&](uint32_t row) -> uint64_t {
-    return static_cast<uint64_t>(M[H.off_rowlab + 2 * row]) |
-           (static_cast<uint64_t>(M[H.off_rowlab + 2 * row + 1]) << 32);
-  };
-  // entry of process q's row for label l (present in the row's mask)
-  auto syn_entry = [&](uint32_t row, uint32_t l) -> uint32_t {
-    const uint64_t below = (l >= 64) ? ~0ull : ((1ull << l) - 1ull);
-    return M[H.off_syn_start + row] + static_cast<uint32_t>(__popcll(rowlab(row) & below));
-  };
+  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
 
   for (uint64_t b0 = gw * 64; b0 < nwork; b0 += nw * 64) {
     const uint64_t si = b0 + lane;
@@ -449,97 +385,15 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
     uint64_t key[W];
 #pragma unroll
     for (int j = 0; j < W; j++) key[j] = has ? front[fi * W + j] : kFlag;
-
-    // ---- pass 1: rows, successor count, enabledness, safety
-    uint64_t labs = 0, cnt = 0;
-    bool anyloc = false, anyen = false;
-    for (int p = 0; p < N; p++) {
-      uint32_t row = kNone;
-      if (has) {
-        const uint32_t sp = kget<W>(key, M[H.off_floc + p]);
-        if (sp < M[H.off_rowcnt + p]) {
-          row = M[H.off_rowbase + p] + sp;
-          if (M[H.off_rowflags + row] & kRowBad) row = kNone;
-        }
-        if (row == kNone) bad = true;
-      }
-      rows[p * 64 + lane] = row;
-      if (row != kNone) {
-        const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
-        cnt += l1 - l0;
-        anyloc |= (M[H.off_rowflags + row] & kRowHasLocal) != 0;
-        labs |= rowlab(row);
-        if (safety && p == N - 1)
-          for (uint32_t t = l0; t < l1; t++)
-            if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) viol = true;
-      }
-    }
-    for (uint64_t L = labs; L; L &= L - 1) {
-      const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
-      uint32_t en = 0;
-      for (int p = 0; p < N; p++) {
-        const uint32_t row = rows[p * 64 + lane];
-        if (row != kNone && ((rowlab(row) >> l) & 1ull)) en |= 1u << p;
-      }
-      const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
-      for (uint32_t j = rs; j < re; j++) {
-        const uint32_t r = M[H.off_rules + j];
-        if (r & ~en) continue;
-        anyen = true;
-        uint64_t prod = 1;
-        for (uint32_t rr = r; rr; rr &= rr - 1) {
-          const int q = __ffs(rr) - 1;
-          prod *= M[H.off_syn_tcnt + syn_entry(rows[q * 64 + lane], l)];
-        }
-        cnt += prod;
-        if (safety && ((r >> (N - 1)) & 1u)) {
-          const uint32_t e = syn_entry(rows[(N - 1) * 64 + lane], l);
-          const uint32_t ts = M[H.off_syn_tstart + e], tc = M[H.off_syn_tcnt + e];
-          for (uint32_t i = 0; i < tc; i++)
-            if ((M[H.off_syn_tgt + ts + i] & fmask_last) == 1u) viol = true;
-        }
-      }
-    }
-    if (a.property == 1 && has && !anyloc && !anyen) viol = true;  // cu:1134-1151
-    if (!has) cnt = 0;
-
-    // ---- reserve candidate space (as in k_expand)
+    const TpsInfo inf = tps_scan<W>(M, H, key, has, rows, lane, safety);
+    viol |= inf.viol;
+    bad |= inf.bad;
+    if (a.property == 1 && inf.dead) viol = true;  // cu:1134-1151
     uint64_t total = 0;
-    const uint64_t off = wave_excl_scan(cnt, &total);
+    const uint64_t off = wave_excl_scan(inf.cnt, &total);
+    const uint64_t left = (nwork - b0 + nw * 64 - 1) / (nw * 64);
     uint64_t obase = 0;
-    int ok = 1;
-    if (total) {
-      if (cfull) {
-        ok = 0;
-      } else if (total > cleft) {
-        release(cbase, cleft);
-        const uint64_t left = (nwork - b0 + nw * 64 - 1) / (nw * 64);
-        uint64_t want = total * left;
-        want = want < kChunk ? want : kChunk;
-        want = want > total ? want : total;
-        uint64_t got = 0;
-        if (lane == 0)
-          got = atomicAdd(reinterpret_cast<unsigned long long *>(&C->cand_count),
-                          static_cast<unsigned long long>(want));
-        got = __shfl(got, 0, 64);
-        const uint64_t avail = got < a.cand_cap ? a.cand_cap - got : 0;
-        if (avail >= total) {
-          cbase = got;
-          cleft = want < avail ? want : avail;
-        } else {
-          release(got, want);
-          cbase = 0;
-          cleft = 0;
-          cfull = true;
-          ok = (total > a.cand_cap) ? -1 : 0;
-        }
-      }
-      if (ok > 0) {
-        obase = cbase;
-        cbase += total;
-        cleft -= total;
-      }
-    }
+    const int ok = ch.reserve<W>(total, left, &obase);
     if (ok <= 0) {
       if (ok < 0 && lane == 0) C->overflow = 2;
       const uint64_t m = ballot(has);
@@ -552,61 +406,16 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
     }
     acc_trans += total;
     acc_states += __popcll(ballot(has));
-
-    // ---- pass 2: emit
-    uint64_t pos = obase + off;
     if (has) {
-      for (int p = 0; p < N; p++) {
-        const uint32_t row = rows[p * 64 + lane];
-        if (row == kNone) continue;
-        const uint32_t fl = M[H.off_floc + p];
-        const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
-        for (uint32_t t = l0; t < l1; t++) {
-          uint64_t s[W];
+      uint64_t pos = obase + off;
+      tps_emit<W>(M, H, key, rows, lane, inf.labs, [&](const uint64_t (&sv)[W]) {
 #pragma unroll
-          for (int j = 0; j < W; j++) s[j] = key[j];
-          kset<W>(s, fl, M[H.off_loc_tgt + t]);
-#pragma unroll
-          for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
-          pos++;
-        }
-      }
-      for (uint64_t L = labs; L; L &= L - 1) {
-        const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
-        uint32_t en = 0;
-        for (int p = 0; p < N; p++) {
-          const uint32_t row = rows[p * 64 + lane];
-          if (row != kNone && ((rowlab(row) >> l) & 1ull)) en |= 1u << p;
-        }
-        const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
-        for (uint32_t j = rs; j < re; j++) {
-          const uint32_t r = M[H.off_rules + j];
-          if (r & ~en) continue;
-          uint32_t prod = 1;
-          for (uint32_t rr = r; rr; rr &= rr - 1)
-            prod *= M[H.off_syn_tcnt + syn_entry(rows[(__ffs(rr) - 1) * 64 + lane], l)];
-          for (uint32_t k = 0; k < prod; k++) {
-            uint64_t s[W];
-#pragma unroll
-            for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
-            uint32_t rem = k;
-            for (uint32_t rr = r; rr; rr &= rr - 1) {  // lowest process varies fastest
-              const int q = __ffs(rr) - 1;
-              const uint32_t e = syn_entry(rows[q * 64 + lane], l);
-              const uint32_t tc = M[H.off_syn_tcnt + e];
-              const uint32_t d = rem % tc;
-              rem /= tc;
-              kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + M[H.off_syn_tstart + e] + d]);
-            }
-#pragma unroll
-            for (int jj = 0; jj < W; jj++) a.cand[pos * W + jj] = s[jj];
-            pos++;
-          }
-        }
-      }
+        for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+        pos++;
+      });
     }
   }
-  release(cbase, cleft);
+  ch.release_all<W>();
   if (lane == 0) {
     if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
                              static_cast<unsigned long long>(acc_trans));

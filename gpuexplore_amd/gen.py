@@ -414,9 +414,12 @@ def config3(seed: int = 3) -> Network:
 
 
 def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, str]]:
-    """Batch of ``count`` models of uneven size: N in 4..14, 4..32 states,
-    out-degree 1..3; even indices run -d, odd ones -p with a seeded bad-state
-    predicate over two processes encoded as a monitor process."""
+    """Batch of ``count`` models of uneven size: N in 4..14 processes, 4..32
+    local states (control states x 2-4 tokens), out-degree 1..3 with kinds
+    drawn from a mixed local / rendezvous / barrier distribution, 1-2 tokens
+    spent per round (1 in models of 12 or more processes).  Even indices run -d; odd ones run -p with a seeded
+    bad-state predicate over two processes, encoded as a two-state monitor in
+    last position."""
     out = []
     for i in range(count):
         rng = Rng(seed0 + i)
@@ -425,13 +428,16 @@ def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, st
         ctrl = max(2, (4 + rng.below(29)) // tokens)
         S = ctrl * tokens
         outdeg = 1 + rng.below(3)
-        mix = (0.6, 0.3, 0.1)
+        budgets = [1 + (rng.below(4) == 0 and tokens > 2 and n < 12) for _ in range(n)]
+        mix = (0.5, 0.35, 0.15)
         if i % 2 == 0:
-            out.append((throttled(n, S, tokens, outdeg, mix, 3, seed0 + i), "-d"))
+            net = rounds(n, S, tokens, budgets, 3, seed0 + i, mix=mix, outdeg=outdeg)
+            out.append((net, "-d"))
         else:
             a, b = rng.sample(n, 2)
             mon = (a, rng.below(S), b, rng.below(S))
-            out.append((throttled(n, S, tokens, outdeg, mix, 3, seed0 + i, monitor=mon), "-p"))
+            net = rounds(n, S, tokens, budgets, 3, seed0 + i, monitor=mon, mix=mix, outdeg=outdeg)
+            out.append((net, "-p"))
     return out
 
 

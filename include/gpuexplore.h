@@ -126,6 +126,45 @@ void gx_explorer_free(gx_explorer *e);
 const char *gx_strerror(int status);
 
 /* ------------------------------------------------------------------ */
+/* Batched exploration: n independent models (plain method, each with its own
+ * property) explored together.  Every instance owns a partition of one
+ * visited set and its own frontiers; one round expands one BFS level of
+ * every unfinished instance, and `levels_per_sync` rounds are captured in a
+ * hipGraph and replayed until all instances stop.  Per instance the result
+ * equals gx_explore's.  No reference counterpart: the reference explores one
+ * model per process (cu:1738-2188). */
+typedef struct gx_batch_options {
+  uint32_t threads;          /* threads per workgroup (0 = 256) */
+  uint32_t blocks;           /* workgroups per launch (0 = auto) */
+  uint32_t levels_per_sync;  /* rounds per graph replay (0 = 1) */
+  uint32_t bucket_bytes;     /* 64, 128 or 256 (0 = auto) */
+  uint64_t q_ints;           /* total visited-set size in 32-bit ints, split over instances */
+  uint64_t hash_seed;
+  uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
+  uint32_t use_graph;        /* 1: replay captured rounds; 0: launch them one by one */
+  uint32_t reserved;
+} gx_batch_options;
+
+typedef struct gx_batch_stats {
+  double t_explore_s;        /* wall time from the first round to the last */
+  double t_init_s;           /* allocation, table init, initial states */
+  uint64_t rounds;           /* rounds run (levels plus deferred rounds) */
+  uint64_t graph_launches;   /* host synchronisations */
+  uint64_t lookups;          /* FIND-OR-PUT operations, all instances */
+  uint64_t states;           /* sum over instances */
+  uint64_t transitions;      /* sum over instances */
+} gx_batch_stats;
+
+void gx_batch_options_default(gx_batch_options *o);
+
+/* weights (optional, n entries): relative visited-set partition sizes, e.g.
+ * estimated state counts; NULL splits the table evenly.  An instance whose
+ * partition fills reports table_full like gx_explore. */
+int gx_explore_batch(const gx_model *const *models, const int32_t *properties,
+                     const uint64_t *weights, int n, const gx_batch_options *o,
+                     gx_result *results, gx_batch_stats *stats);
+
+/* ------------------------------------------------------------------ */
 /* Visited-set access (FIND-OR-PUT on its own), the hash-insert phase of the
  * exploration (cu:361-450 FINDORPUT_WARP, cu:656-704 init/count).  Keys are
  * `key_words` 64-bit words whose bit 63 is reserved (set by the library). */

@@ -69,6 +69,15 @@ def test_generated_models_compile(gx):
         assert i["key_words"] == (i["state_bits"] + 62) // 63
 
 
+def test_batch_without_device_reports_status(gx):
+    if gx.device_count() > 0:
+        pytest.skip("a device is visible")
+    m = gx.Model.load(os.path.join(GOLDEN, "tiny2"))
+    with pytest.raises(gx.GxError) as e:
+        gx.explore_batch([m, m], [0, 1])
+    assert e.value.status == -6
+
+
 def test_explore_without_device_reports_status(gx):
     if gx.device_count() > 0:
         pytest.skip("a device is visible")
