@@ -43,7 +43,7 @@ def batched_vs_plain(gx, gen, tmp: str) -> dict:
         paths.append(p)
         props.append(gx.PROP_DEADLOCK if flag == "-d" else gx.PROP_SAFETY)
     models = [gx.Model.load(p) for p in paths]
-    q_batch = 64 << 22
+    q_batch = 64 << 24  # 64 MiB of visited set per instance
     tuned = autotune.tune_batch(models, props, q_batch, repeats=2)
     best = tuned["best"]
     runs = []
@@ -52,13 +52,15 @@ def batched_vs_plain(gx, gen, tmp: str) -> dict:
         runs.append(stats)
     t_batch = min(s["t_explore_s"] for s in runs)
     states = runs[0]["states"]
-    explorers = [gx.Explorer(m, property=pr, q_ints=1 << 22) for m, pr in zip(models, props)]
+    if any(r["table_full"] for r in res):
+        raise RuntimeError("a batch partition filled up")
+    explorers = [gx.Explorer(m, property=pr, q_ints=1 << 24) for m, pr in zip(models, props)]
     plain_t = []
     for _ in range(2):
         t = 0.0
         for i, ex in enumerate(explorers):
             r = ex.run()
-            if (r["states"], r["transitions"]) != (res[i]["states"], res[i]["transitions"]):
+            if r["table_full"] or (r["states"], r["transitions"]) != (res[i]["states"], res[i]["transitions"]):
                 raise RuntimeError(f"batched and plain counts differ for model {i}")
             t += r["t_explore_s"]
         plain_t.append(t)

@@ -150,8 +150,15 @@ __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
   ch.release_all<W>();
 }
 
+// New keys are staged per wave in LDS together with their instance and
+// appended to that instance's next frontier in blocks, one atomicAdd per
+// block: candidates of one instance come in long runs, and a per-key atomic
+// on one instance's counter would serialise the device.
+constexpr int kBatchStage = 128;
+
 template <int W, int BW>
 __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
+  extern __shared__ uint64_t bstage_all[];
   BatchCtrl *C = a.ctrl;
   if (C->active == 0) return;
   const uint64_t n = C->cand_count < a.cand_cap ? C->cand_count : a.cand_cap;
@@ -163,9 +170,34 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
   const int sub = lane % LPB;
   const int cg = lane / LPB;
   const int gbase = cg * LPB;
+  uint64_t *stage = bstage_all + static_cast<size_t>(threadIdx.x >> 6) * (kBatchStage * W);
+  uint32_t staged = 0;          // wave-uniform
+  uint32_t stage_inst = 0;      // instance of the staged keys
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint64_t acc_look = 0;
+  auto flush = [&]() {
+    if (!staged) return;
+    wave_sync();
+    BatchInst *I = &a.inst[stage_inst];
+    const uint32_t nxt = I->cur ^ 1u;
+    uint64_t ob = 0;
+    if (lane == 0)
+      ob = atomicAdd(reinterpret_cast<unsigned long long *>(&I->front_count[nxt]),
+                     static_cast<unsigned long long>(staged));
+    ob = __shfl(ob, 0, 64);
+    bool over = false;
+    for (uint32_t k = lane; k < staged * W; k += 64) {
+      const uint64_t idx = ob + k / W;
+      if (idx < I->fcap)
+        a.front[nxt][(I->fo + ob) * W + k] = stage[k];
+      else
+        over = true;
+    }
+    if (ballot(over) && lane == 0) I->overflow = 1;
+    wave_sync();
+    staged = 0;
+  };
   for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
     const uint64_t ci = c0 + cg;
     bool active = ci < n;
@@ -174,28 +206,33 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
     for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
     active = active && key[0] != 0;
     const uint32_t i = active ? a.cand_inst[ci] : 0u;
-    BatchInst *I = &a.inst[i];
+    const BatchInst *I = &a.inst[i];
     Table T{a.table + (active ? I->tb : 0) * BW, active ? I->nb : 1, a.seed};
     const int st = find_or_put_group<W, BW>(T, key, fmt, key_hash<W>(key, a.seed), sub, gbase, active);
     acc_look += __popcll(ballot(active && sub == 0));
     if (active && sub == 0) {
-      if (st == kInserted) {
-        const uint32_t nxt = I->cur ^ 1u;
-        const uint64_t idx =
-            atomicAdd(reinterpret_cast<unsigned long long *>(&I->front_count[nxt]), 1ull);
-        if (idx < I->fcap) {
+      if (st == kFull) a.inst[i].table_full = 1;
+      if (st == kTimeout) a.inst[i].spin_timeout = 1;
+    }
+    const bool is_new = active && st == kInserted;
+    uint64_t m = ballot(is_new && sub == 0);
+    while (m) {  // one group per instance present in this iteration
+      const uint32_t i0 = __shfl(i, __ffsll(static_cast<unsigned long long>(m)) - 1, 64);
+      const uint64_t mm = ballot(is_new && sub == 0 && i == i0);
+      const uint32_t cnt = __popcll(mm);
+      if (staged && (stage_inst != i0 || staged + cnt > static_cast<uint32_t>(kBatchStage))) flush();
+      stage_inst = i0;
+      if (is_new && i == i0) {
+        const uint32_t pos = staged + __popcll(mm & ((1ull << gbase) - 1ull));
 #pragma unroll
-          for (int j = 0; j < W; j++) a.front[nxt][(I->fo + idx) * W + j] = key[j];
-        } else {
-          I->overflow = 1;
-        }
-      } else if (st == kFull) {
-        I->table_full = 1;
-      } else if (st == kTimeout) {
-        I->spin_timeout = 1;
+        for (int j = 0; j < W; j++)
+          if (j % LPB == sub) stage[pos * W + j] = key[j];
       }
+      staged += cnt;
+      m &= ~mm;
     }
   }
+  flush();
   if (lane == 0 && acc_look)
     atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(acc_look));
 }
@@ -208,7 +245,12 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   if (C->active == 0) return;
   const uint32_t tid = threadIdx.x;
   const bool deferred = C->deferred_count[C->deferred_cur ^ 1u] > 0;
+  const bool fatal = C->cand_overflow != 0;  // a chunk that can never fit: stop everything
   __syncthreads();
+  if (fatal) {
+    if (tid == 0) C->active = 0;
+    return;
+  }
   if (tid == 0) {
     C->cand_count = 0;
     C->rounds += 1;
@@ -270,10 +312,11 @@ template <int W>
 static hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, hipStream_t s) {
   const size_t shm = static_cast<size_t>(threads / 64) * a.max_procs * 64 * 4;
   hipLaunchKernelGGL((k_batch_expand<W>), dim3(blocks), dim3(threads), shm, s, a);
+  const size_t ishm = static_cast<size_t>(threads / 64) * kBatchStage * W * 8;
   switch (BW) {
-    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(blocks), dim3(threads), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(blocks), dim3(threads), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(blocks), dim3(threads), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(blocks), dim3(threads), ishm, s, a); break;
+    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(blocks), dim3(threads), ishm, s, a); break;
+    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(blocks), dim3(threads), ishm, s, a); break;
     default: return hipErrorInvalidValue;
   }
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
@@ -285,9 +328,9 @@ static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStrea
 #define GX_BI(Wk)                                                                               \
   case Wk:                                                                                      \
     switch (BW) {                                                                               \
-      case 8: hipLaunchKernelGGL((k_batch_insert<Wk, 8>), dim3(64), dim3(256), 0, s, a); break;   \
-      case 16: hipLaunchKernelGGL((k_batch_insert<Wk, 16>), dim3(64), dim3(256), 0, s, a); break; \
-      case 32: hipLaunchKernelGGL((k_batch_insert<Wk, 32>), dim3(64), dim3(256), 0, s, a); break; \
+      case 8: hipLaunchKernelGGL((k_batch_insert<Wk, 8>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break;   \
+      case 16: hipLaunchKernelGGL((k_batch_insert<Wk, 16>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break; \
+      case 32: hipLaunchKernelGGL((k_batch_insert<Wk, 32>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break; \
       default: return hipErrorInvalidValue;                                                     \
     }                                                                                           \
     break;
@@ -416,7 +459,7 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   }
   const uint64_t front_total = fo;
   uint64_t cand_cap = o->cand_capacity ? o->cand_capacity : std::max<uint64_t>(1u << 22, front_total);
-  cand_cap = std::min<uint64_t>(cand_cap, 0xFFFFFFFFull);
+  cand_cap = std::max<uint64_t>(std::min<uint64_t>(cand_cap, 0xFFFFFFFFull), static_cast<uint64_t>(n));
   const uint64_t max_chunks = front_total / 64 + static_cast<uint64_t>(n) + 1;
 
   const auto t_init0 = std::chrono::steady_clock::now();

@@ -35,13 +35,14 @@ def batch_models(tmp_path_factory, oracle_mod):
 def run_batch(gx, batch_models, **kw):
     models = [gx.Model.load(p) for p, _, _ in batch_models]
     props = [prop for _, prop, _ in batch_models]
-    kw.setdefault("q_ints", 64 << 22)
+    kw.setdefault("q_ints", 64 << 24)  # 64 MiB per instance
     res, stats, rc = gx.explore_batch(models, props, **kw)
     return res, stats, rc
 
 
 def check(batch_models, res):
     for (p, prop, o), r in zip(batch_models, res):
+        assert r["table_full"] == 0, (p, "partition too small")
         got = (r["states"], r["transitions"], r["levels"], bool(r["violation"]))
         want = (o["states"], o["transitions"], o["levels"], o["violation"])
         assert got == want, (p, prop, got, want)
