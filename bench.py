@@ -91,6 +91,9 @@ def main() -> int:
     ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the Poisson job stream through the server (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="jobs per second of the stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -153,6 +156,13 @@ def main() -> int:
 
     batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp)
 
+    serving = None
+    if args.serve_seconds > 0:
+        from gpuexplore_amd import serve
+        serving = serve.serve_workload(args.serve_rate, args.serve_seconds, seed=0,
+                                       log=lambda m: print(m, file=sys.stderr, flush=True))
+        serving["workload"] = "config5: Poisson stream of gen.config5_job models, deadlock check"
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
@@ -212,6 +222,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched_vs_plain": batch,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -99,6 +99,48 @@ class BatchStats(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("key_words", ctypes.c_uint32),
+        ("max_procs", ctypes.c_uint32),
+        ("threads", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32),
+        ("rounds_per_step", ctypes.c_uint32),
+        ("table_bytes", ctypes.c_uint64),
+        ("page_bytes", ctypes.c_uint64),
+        ("cand_capacity", ctypes.c_uint64),
+        ("hash_seed", ctypes.c_uint64),
+        ("model_words", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class ServerStatus(ctypes.Structure):
+    _fields_ = [
+        ("running", ctypes.c_uint32),
+        ("waiting", ctypes.c_uint32),
+        ("finished_pending", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("steps", ctypes.c_uint64),
+        ("rounds", ctypes.c_uint64),
+        ("pages_used", ctypes.c_uint64),
+        ("pages_total", ctypes.c_uint64),
+    ]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [
+        ("job_id", ctypes.c_uint64),
+        ("result", Result),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("status", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
+    ]
+
+
 class ModelInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in (
         "nr_procs", "bits_act", "proc_nrstates", "bits_statevector", "sv_nints", "liveness",
@@ -119,6 +161,12 @@ SIGNATURES = {
     "gx_explorer_run": (ctypes.c_int, [_P, ctypes.POINTER(Result)]),
     "gx_explorer_free": (None, [_P]),
     "gx_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
+    "gx_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),
+    "gx_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
+    "gx_server_submit": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "gx_server_step": (ctypes.c_int, [_P, ctypes.POINTER(ServerStatus)]),
+    "gx_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(JobResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gx_server_free": (None, [_P]),
     "gx_explore_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int32),
                                         ctypes.POINTER(ctypes.c_uint64), ctypes.c_int,
                                         ctypes.POINTER(BatchOptions), ctypes.POINTER(Result),
@@ -262,6 +310,51 @@ def explore_batch(models, properties=None, weights=None, **kw):
     if rc not in (GX_OK, -10):
         raise GxError(rc, "explore_batch")
     return [r.as_dict() for r in res], {k: getattr(st, k) for k, _ in st._fields_}, rc
+
+
+class Server:
+    """gx_server: continuous batching of model-checking jobs."""
+
+    def __init__(self, **kw):
+        o = ServerOptions()
+        lib().gx_server_options_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        h = ctypes.c_void_p()
+        _check(lib().gx_server_create(ctypes.byref(o), ctypes.byref(h)), "server")
+        self.h = h
+        self._models = {}  # job id -> Model (kept alive until the job is polled)
+
+    def submit(self, model: Model, prop: int = 0, est_states: int = 1) -> int:
+        jid = ctypes.c_uint64(0)
+        _check(lib().gx_server_submit(self.h, model.h, prop, est_states, ctypes.byref(jid)), "submit")
+        self._models[jid.value] = model
+        return jid.value
+
+    def step(self) -> dict:
+        st = ServerStatus()
+        rc = lib().gx_server_step(self.h, ctypes.byref(st))
+        _check(rc, "step")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def poll(self, max_results: int = 4096) -> list:
+        buf = (JobResult * max_results)()
+        n = ctypes.c_int(0)
+        _check(lib().gx_server_poll(self.h, buf, max_results, ctypes.byref(n)), "poll")
+        out = []
+        for i in range(n.value):
+            j = buf[i]
+            d = j.result.as_dict()
+            d.update(job_id=j.job_id, t_submit=j.t_submit, t_admit=j.t_admit, t_done=j.t_done,
+                     status=j.status, restarts=j.restarts)
+            self._models.pop(j.job_id, None)
+            out.append(d)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.gx_server_free(self.h)
+            self.h = None
 
 
 class Table:

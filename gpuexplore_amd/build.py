@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter",
 # packed FP32 instructions stay disabled in all device code
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["gx_model.cpp", "gx_host.cpp", "gx_kernels.hip", "gx_batch.hip"]
+SOURCES = ["gx_model.cpp", "gx_host.cpp", "gx_kernels.hip", "gx_batch.hip", "gx_serve.cpp"]
 
 
 def _run(cmd):

@@ -1,0 +1,63 @@
+// gx_batch.h -- device structures of the multi-instance path (batching and
+// serving) and the launches of one round.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gx_common.h"
+
+namespace gx {
+
+// One instance (model exploration) of a batch or a server slot.
+struct BatchInst {
+  ModelHdr hdr;
+  uint32_t model_off;   // into the concatenated model buffer (words)
+  int32_t property;
+  uint64_t tb, nb;      // visited-set partition: first bucket, buckets
+  uint64_t fo, fcap;    // frontier region: first entry, capacity
+  uint64_t front_count[2];
+  uint64_t inserts, transitions, expanded;
+  uint64_t chunk_prefix;  // first chunk of this instance in the round
+  uint32_t cur, level;
+  uint32_t violation, table_full, overflow, bad_state, spin_timeout, stop;
+};
+
+struct BatchCtrl {
+  uint64_t cand_count;
+  uint64_t lookups;
+  uint64_t total_chunks;
+  uint64_t deferred_count[2];
+  uint32_t deferred_cur;
+  uint32_t use_deferred;
+  uint32_t active;       // instances not stopped
+  uint32_t rounds;
+  uint32_t n;
+  uint32_t cand_overflow;
+  uint32_t prefix_only;  // next advance only recomputes the chunk prefix
+  uint32_t pad;
+};
+
+struct BatchArgs {
+  BatchInst *inst;
+  BatchCtrl *ctrl;
+  const uint32_t *models;
+  uint64_t *front[2];   // pools, W words per entry
+  uint64_t *cand;
+  uint32_t *cand_inst;
+  uint64_t cand_cap;
+  uint64_t *deferred[2];  // chunk descriptors: instance << 40 | first state
+  uint64_t *table;
+  uint64_t seed;
+  uint32_t max_procs;
+};
+
+// expand + FIND-OR-PUT + advance of every unfinished instance (one BFS level)
+hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads, hipStream_t s);
+// the advance step alone (with prefix_only set: chunk prefix after admissions)
+hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s);
+// the initial state into a zeroed partition (first bucket tb, nb buckets)
+// and as entry fo of frontier pool front[0]
+hipError_t launch_place_initial(const BatchArgs &a, int W, int BW, uint64_t tb, uint64_t nb, uint64_t fo,
+                                hipStream_t s);
+
+}  // namespace gx

@@ -23,48 +23,9 @@
 #include "gx_kernels.h"
 #include "gx_model.h"
 #include "gx_tps.h"
+#include "gx_batch.h"
 
 namespace gx {
-
-struct BatchInst {
-  ModelHdr hdr;
-  uint32_t model_off;   // into the concatenated model buffer (words)
-  int32_t property;
-  uint64_t tb, nb;      // table partition: first bucket, buckets
-  uint64_t fo, fcap;    // frontier region: first entry, capacity
-  uint64_t front_count[2];
-  uint64_t inserts, transitions, expanded;
-  uint64_t chunk_prefix;  // first chunk of this instance in the round
-  uint32_t cur, level;
-  uint32_t violation, table_full, overflow, bad_state, spin_timeout, stop;
-};
-
-struct BatchCtrl {
-  uint64_t cand_count;
-  uint64_t lookups;
-  uint64_t total_chunks;
-  uint64_t deferred_count[2];
-  uint32_t deferred_cur;
-  uint32_t use_deferred;
-  uint32_t active;       // instances not stopped
-  uint32_t rounds;
-  uint32_t n;
-  uint32_t cand_overflow;
-};
-
-struct BatchArgs {
-  BatchInst *inst;
-  BatchCtrl *ctrl;
-  const uint32_t *models;
-  uint64_t *front[2];   // pools, W words per entry
-  uint64_t *cand;
-  uint32_t *cand_inst;
-  uint64_t cand_cap;
-  uint64_t *deferred[2];  // chunk descriptors: instance << 40 | first state
-  uint64_t *table;
-  uint64_t seed;
-  uint32_t max_procs;
-};
 
 __device__ __forceinline__ uint32_t find_instance(const BatchInst *I, uint32_t n, uint64_t c) {
   uint32_t lo = 0, hi = n;  // last instance with chunk_prefix <= c
@@ -242,10 +203,11 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   __shared__ uint64_t scan[1024];
   __shared__ uint32_t nact;
   BatchCtrl *C = a.ctrl;
-  if (C->active == 0) return;
+  if (C->active == 0 && C->prefix_only == 0) return;
   const uint32_t tid = threadIdx.x;
   const bool deferred = C->deferred_count[C->deferred_cur ^ 1u] > 0;
   const bool fatal = C->cand_overflow != 0;  // a chunk that can never fit: stop everything
+  const bool prefix_only = C->prefix_only != 0;  // recompute chunks after admissions
   __syncthreads();
   if (fatal) {
     if (tid == 0) C->active = 0;
@@ -253,9 +215,9 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   }
   if (tid == 0) {
     C->cand_count = 0;
-    C->rounds += 1;
+    if (!prefix_only) C->rounds += 1;
   }
-  if (deferred) {  // the level is not complete: run the deferred chunks
+  if (deferred && !prefix_only) {  // the level is not complete: run the deferred chunks
     if (tid == 0) {
       const uint32_t dc = C->deferred_cur;
       C->deferred_count[dc] = 0;
@@ -277,7 +239,9 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
     uint64_t chunks = 0;
     if (i < n) {
       BatchInst *I = &a.inst[i];
-      if (!I->stop) {
+      if (!I->stop && prefix_only) {
+        chunks = (I->front_count[I->cur] + 63) / 64;
+      } else if (!I->stop) {
         const uint32_t cur = I->cur, nxt = cur ^ 1u;
         const uint64_t fresh = I->front_count[nxt] < I->fcap ? I->front_count[nxt] : I->fcap;
         I->inserts += fresh;
@@ -305,11 +269,12 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   if (tid == 0) {
     C->total_chunks = carry;
     C->active = nact;
+    C->prefix_only = 0;
   }
 }
 
 template <int W>
-static hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, hipStream_t s) {
+hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, hipStream_t s) {
   const size_t shm = static_cast<size_t>(threads / 64) * a.max_procs * 64 * 4;
   hipLaunchKernelGGL((k_batch_expand<W>), dim3(blocks), dim3(threads), shm, s, a);
   const size_t ishm = static_cast<size_t>(threads / 64) * kBatchStage * W * 8;
@@ -347,7 +312,7 @@ static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStrea
   return hipGetLastError();
 }
 
-static hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
+hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
                                  hipStream_t s) {
   switch (W) {
     case 1: return launch_round<1>(a, BW, blocks, threads, s);
@@ -357,6 +322,44 @@ static hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blo
     case 5: return launch_round<5>(a, BW, blocks, threads, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// The initial state (all processes in local state 0, cu:668-682) goes to
+// slot 0 of its first probe bucket: the partition was zeroed just before in
+// stream order, so that is where FIND-OR-PUT would have put it.
+template <int W>
+__global__ void k_place_initial(BatchArgs a, uint64_t tb, uint64_t nb, uint64_t fo, uint32_t BW) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t key[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) key[j] = kFlag;
+  const uint64_t b = bucket_of(key_hash<W>(key, a.seed), 0, nb);
+  uint64_t *slot0 = a.table + (tb + b) * BW;
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    slot0[j] = key[j];
+    a.front[0][fo * W + j] = key[j];
+  }
+}
+
+hipError_t launch_place_initial(const BatchArgs &a, int W, int BW, uint64_t tb, uint64_t nb, uint64_t fo,
+                                hipStream_t s) {
+  if (nb == 0) return hipErrorInvalidValue;
+  const uint32_t bw = static_cast<uint32_t>(BW);
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_place_initial<1>, dim3(1), dim3(64), 0, s, a, tb, nb, fo, bw); break;
+    case 2: hipLaunchKernelGGL(k_place_initial<2>, dim3(1), dim3(64), 0, s, a, tb, nb, fo, bw); break;
+    case 3: hipLaunchKernelGGL(k_place_initial<3>, dim3(1), dim3(64), 0, s, a, tb, nb, fo, bw); break;
+    case 4: hipLaunchKernelGGL(k_place_initial<4>, dim3(1), dim3(64), 0, s, a, tb, nb, fo, bw); break;
+    case 5: hipLaunchKernelGGL(k_place_initial<5>, dim3(1), dim3(64), 0, s, a, tb, nb, fo, bw); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
+  return hipGetLastError();
 }
 
 }  // namespace gx

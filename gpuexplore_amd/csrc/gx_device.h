@@ -18,7 +18,7 @@
 
 namespace gx {
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x ^= x >> 33;
   x *= 0xff51afd7ed558ccdull;
   x ^= x >> 33;
@@ -28,7 +28,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 }
 
 template <int W>
-__device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[W], uint64_t seed) {
+__host__ __device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[W], uint64_t seed) {
   uint64_t h = seed ^ 0x9e3779b97f4a7c15ull;
 #pragma unroll
   for (int j = 0; j < W; j++) h = mix64(h ^ k[j]) + static_cast<uint64_t>(j);
@@ -36,9 +36,13 @@ __device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[W], uint64_t se
 }
 
 // bucket of probe i: multiply-shift reduction of an independent 64-bit hash
-__device__ __forceinline__ uint64_t bucket_of(uint64_t h, int i, uint64_t nbuckets) {
+__host__ __device__ __forceinline__ uint64_t bucket_of(uint64_t h, int i, uint64_t nbuckets) {
   uint64_t hi = mix64(h + static_cast<uint64_t>(i + 1) * 0x9e3779b97f4a7c15ull);
+#if defined(__HIP_DEVICE_COMPILE__)
   return __umul64hi(hi, nbuckets);
+#else
+  return static_cast<uint64_t>((static_cast<unsigned __int128>(hi) * nbuckets) >> 64);
+#endif
 }
 
 // ---------------------------------------------------------------------------

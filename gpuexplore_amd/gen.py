@@ -441,6 +441,21 @@ def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, st
     return out
 
 
+def config5_job(seed: int) -> Tuple[Network, str, int]:
+    """One job of the serving stream: 90% drawn from the config-3 generator
+    (phased rounds, 8 local states) with 6..12 processes, 10% from the
+    config-2 generator (16 local states, token budgets) with 12..14
+    processes.  Returns (network, flag, estimated states); the estimate is
+    deliberately rough (the server restarts a job whose partition fills)."""
+    rng = Rng(0xC5 * 1000003 + seed)
+    if rng.below(10) == 0:
+        n = 12 + rng.below(3)
+        budgets = [2 if p % 5 == 0 else 1 for p in range(n)]
+        return rounds(n, 16, 4, budgets, 4, seed), "-d", int(2.5 ** n)
+    n = 6 + rng.below(7)
+    return rounds(n, 8, 4, [1] * n, 4, seed, mix=(0.7, 0.2, 0.1), phased=True), "-d", int(3.0 ** n)
+
+
 # ----------------------------------------------------------------------
 # Closed-form families (used to pin the oracle)
 # ----------------------------------------------------------------------

@@ -157,6 +157,58 @@ typedef struct gx_batch_stats {
 
 void gx_batch_options_default(gx_batch_options *o);
 
+/* ------------------------------------------------------------------ */
+/* Serving: a continuous-batching request loop.  Jobs (model + property +
+ * estimated state count) are queued by gx_server_submit; every
+ * gx_server_step admits waiting jobs while an instance slot and a
+ * contiguous run of visited-set pages for the job's estimated footprint are
+ * free, replays `rounds_per_step` BFS rounds of all running jobs, and
+ * retires finished ones.  A job whose partition fills up is restarted with
+ * twice the pages, so results equal gx_explore's whatever the estimate.
+ * Visited sets and frontiers of running jobs stay on the device between
+ * steps.  No reference counterpart (the reference runs one model per
+ * process). */
+typedef struct gx_server gx_server;
+
+typedef struct gx_server_options {
+  uint32_t slots;            /* concurrent jobs (0 = 256) */
+  uint32_t key_words;        /* key width of the pool: models up to 63*key_words state bits */
+  uint32_t max_procs;        /* largest N accepted */
+  uint32_t threads;          /* threads per workgroup */
+  uint32_t blocks;           /* workgroups per launch (0 = auto) */
+  uint32_t rounds_per_step;  /* BFS rounds per step (one graph replay) */
+  uint64_t table_bytes;      /* visited-set pool */
+  uint64_t page_bytes;       /* allocation granule of the pool */
+  uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
+  uint64_t hash_seed;
+  uint32_t model_words;      /* largest compiled model accepted (32-bit words) */
+  uint32_t reserved;
+} gx_server_options;
+
+typedef struct gx_server_status {
+  uint32_t running;          /* jobs on the device */
+  uint32_t waiting;          /* jobs queued */
+  uint32_t finished_pending; /* results not yet polled */
+  uint32_t reserved;
+  uint64_t steps, rounds, pages_used, pages_total;
+} gx_server_status;
+
+typedef struct gx_job_result {
+  uint64_t job_id;
+  gx_result result;          /* states, transitions, levels, violation, table_full */
+  double t_submit, t_admit, t_done;  /* steady-clock seconds */
+  int32_t status;            /* GX_OK or an error of this job */
+  int32_t restarts;          /* re-admissions after a full partition */
+} gx_job_result;
+
+void gx_server_options_default(gx_server_options *o);
+int gx_server_create(const gx_server_options *o, gx_server **out);
+int gx_server_submit(gx_server *s, const gx_model *m, int32_t property, uint64_t est_states,
+                     uint64_t *job_id);
+int gx_server_step(gx_server *s, gx_server_status *status);
+int gx_server_poll(gx_server *s, gx_job_result *out, int max, int *n);
+void gx_server_free(gx_server *s);
+
 /* weights (optional, n entries): relative visited-set partition sizes, e.g.
  * estimated state counts; NULL splits the table evenly.  An instance whose
  * partition fills reports table_full like gx_explore. */

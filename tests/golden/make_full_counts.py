@@ -1,0 +1,56 @@
+"""Writes tests/golden/<workload>_counts.json: the oracle's counts for a
+full-size generated workload, with the SHA-256 of the model text so a change
+of the generator is caught before a count comparison is made.
+
+    python tests/golden/make_full_counts.py config2 [--flags -d ...]
+
+A full-size run of the oracle takes tens of minutes (config2: ~35 min and
+~10 GB of host memory on one core); the GPU tests compare against the JSON.
+"""
+
+import argparse
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from gpuexplore_amd import gen  # noqa: E402
+
+WORKLOADS = {"config2": lambda: gen.config2(1), "config3": lambda: gen.config3(3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=sorted(WORKLOADS))
+    ap.add_argument("--flags", nargs="*", default=[])
+    args = ap.parse_args()
+    net = WORKLOADS[args.workload]()
+    text = net.to_gpf()
+    cli = os.path.join(ROOT, "oracle", "_build", "gx_oracle_cli")
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    with tempfile.TemporaryDirectory() as d:
+        base = os.path.join(d, args.workload)
+        with open(base + ".gpf", "w") as f:
+            f.write(text)
+        out = subprocess.run([cli, base] + args.flags, check=True, capture_output=True, text=True).stdout
+    kv = dict(tok.split("=") for tok in out.split())
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"{args.workload}_counts.json")
+    data = json.load(open(path)) if os.path.exists(path) else {}
+    data["gpf_sha256"] = hashlib.sha256(text.encode()).hexdigest()
+    data.setdefault("runs", {})[" ".join(args.flags) or "plain"] = {
+        "states": int(kv["states"]), "transitions": int(kv["transitions"]),
+        "levels": int(kv["levels"]), "violation": bool(int(kv["violation"])),
+        "oracle_seconds": float(kv["seconds"])}
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+    print(path, data)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

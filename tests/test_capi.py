@@ -85,3 +85,18 @@ def test_explore_without_device_reports_status(gx):
     with pytest.raises(gx.GxError) as e:
         gx.explore(m)
     assert e.value.status == -6
+
+
+def test_server_without_device_reports_status(gx):
+    if gx.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(gx.GxError) as e:
+        gx.Server(slots=4)
+    assert e.value.status == -6
+
+
+def test_server_options_default(gx):
+    o = gx.ServerOptions()
+    gx.lib().gx_server_options_default(o)
+    assert o.slots == 256 and o.key_words == 2 and o.rounds_per_step >= 1
+    assert o.table_bytes % o.page_bytes == 0
