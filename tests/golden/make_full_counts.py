@@ -28,6 +28,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("workload", choices=sorted(WORKLOADS))
     ap.add_argument("--flags", nargs="*", default=[])
+    ap.add_argument("--max-states", type=int, default=0,
+                    help="stop after the first level that reaches this many states")
     args = ap.parse_args()
     net = WORKLOADS[args.workload]()
     text = net.to_gpf()
@@ -37,14 +39,20 @@ def main() -> int:
         base = os.path.join(d, args.workload)
         with open(base + ".gpf", "w") as f:
             f.write(text)
-        out = subprocess.run([cli, base] + args.flags, check=True, capture_output=True, text=True).stdout
+        extra = ["--max-states", str(args.max_states)] if args.max_states else []
+        out = subprocess.run([cli, base] + args.flags + extra, check=True, capture_output=True,
+                             text=True).stdout
     kv = dict(tok.split("=") for tok in out.split())
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"{args.workload}_counts.json")
     data = json.load(open(path)) if os.path.exists(path) else {}
     data["gpf_sha256"] = hashlib.sha256(text.encode()).hexdigest()
-    data.setdefault("runs", {})[" ".join(args.flags) or "plain"] = {
+    name = " ".join(args.flags) or "plain"
+    if args.max_states:
+        name += f" max_states={args.max_states}"
+    data.setdefault("runs", {})[name] = {
         "states": int(kv["states"]), "transitions": int(kv["transitions"]),
         "levels": int(kv["levels"]), "violation": bool(int(kv["violation"])),
+        "truncated": bool(int(kv["truncated"])), "max_states": args.max_states,
         "oracle_seconds": float(kv["seconds"])}
     with open(path, "w") as f:
         json.dump(data, f, indent=1)
