@@ -46,7 +46,7 @@ class Options(ctypes.Structure):
         ("frontier_capacity", ctypes.c_uint64),
         ("max_states", ctypes.c_uint64),
         ("expand_kernel", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("insert_blocks", ctypes.c_uint32),
     ]
 
 

@@ -52,7 +52,8 @@ struct BatchArgs {
 };
 
 // expand + FIND-OR-PUT + advance of every unfinished instance (one BFS level)
-hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads, hipStream_t s);
+hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
+                          uint32_t ins_blocks, hipStream_t s);
 // the advance step alone (with prefix_only set: chunk prefix after admissions)
 hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s);
 // the initial state into a zeroed partition (first bucket tb, nb buckets)

@@ -274,14 +274,16 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
 }
 
 template <int W>
-hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, hipStream_t s) {
+hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, uint32_t ins_blocks,
+                        hipStream_t s) {
   const size_t shm = static_cast<size_t>(threads / 64) * a.max_procs * 64 * 4;
   hipLaunchKernelGGL((k_batch_expand<W>), dim3(blocks), dim3(threads), shm, s, a);
-  const size_t ishm = static_cast<size_t>(threads / 64) * kBatchStage * W * 8;
+  const uint32_t it = kInsertThreads;
+  const size_t ishm = static_cast<size_t>(it / 64) * kBatchStage * W * 8;
   switch (BW) {
-    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(blocks), dim3(threads), ishm, s, a); break;
-    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(blocks), dim3(threads), ishm, s, a); break;
-    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(blocks), dim3(threads), ishm, s, a); break;
+    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
+    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
+    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
     default: return hipErrorInvalidValue;
   }
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
@@ -313,13 +315,13 @@ static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStrea
 }
 
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
-                                 hipStream_t s) {
+                          uint32_t ins_blocks, hipStream_t s) {
   switch (W) {
-    case 1: return launch_round<1>(a, BW, blocks, threads, s);
-    case 2: return launch_round<2>(a, BW, blocks, threads, s);
-    case 3: return launch_round<3>(a, BW, blocks, threads, s);
-    case 4: return launch_round<4>(a, BW, blocks, threads, s);
-    case 5: return launch_round<5>(a, BW, blocks, threads, s);
+    case 1: return launch_round<1>(a, BW, blocks, threads, ins_blocks, s);
+    case 2: return launch_round<2>(a, BW, blocks, threads, ins_blocks, s);
+    case 3: return launch_round<3>(a, BW, blocks, threads, ins_blocks, s);
+    case 4: return launch_round<4>(a, BW, blocks, threads, ins_blocks, s);
+    case 5: return launch_round<5>(a, BW, blocks, threads, ins_blocks, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -440,6 +442,7 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   hipDeviceProp_t prop;
   BX_TRY(hipGetDeviceProperties(&prop, dev));
   const uint32_t blocks = o->blocks ? o->blocks : prop.multiProcessorCount * (2048 / threads);
+  const uint32_t ins_blocks = prop.multiProcessorCount * kInsertBlocksPerCU;
   // table partitions proportional to the weights (equal without weights)
   // default: 2^22 ints (16 MiB) per instance
   const uint64_t q_ints = o->q_ints ? o->q_ints : (static_cast<uint64_t>(n) << 22);
@@ -519,7 +522,8 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   if (o->use_graph) {
     BX_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t ce = hipSuccess;
-    for (uint32_t r = 0; r < k && ce == hipSuccess; r++) ce = launch_round_w(a, W, BW, blocks, threads, st);
+    for (uint32_t r = 0; r < k && ce == hipSuccess; r++)
+      ce = launch_round_w(a, W, BW, blocks, threads, ins_blocks, st);
     hipError_t ee = hipStreamEndCapture(st, &graph);
     BX_TRY(ce);
     BX_TRY(ee);
@@ -539,7 +543,7 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
     if (exec) {
       BX_TRY(hipGraphLaunch(exec, st));
     } else {
-      for (uint32_t r = 0; r < k; r++) BX_TRY(launch_round_w(a, W, BW, blocks, threads, st));
+      for (uint32_t r = 0; r < k; r++) BX_TRY(launch_round_w(a, W, BW, blocks, threads, ins_blocks, st));
     }
     launches++;
     BX_TRY(hipMemcpyAsync(&ch, d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));

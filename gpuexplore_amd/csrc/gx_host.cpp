@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <fstream>
 #include <memory>
 #include <sstream>
@@ -70,6 +71,17 @@ struct Events {
     return ev[used++];
   }
 };
+
+// FIND-OR-PUT grid for a level of about front * degree candidates: enough
+// waves for ~8 iterations of 16 candidates each, between one and
+// kInsertBlocksPerCU workgroups per CU (tiny levels must not pay for
+// dispatching thousands of empty workgroups)
+uint32_t insert_grid(uint32_t cus, double front, double degree) {
+  const double cand = front * std::max(1.0, degree);
+  const double waves = cand / (16.0 * 8.0) + 1.0;
+  const double blocks = std::ceil(waves / (kInsertThreads / 64));
+  return static_cast<uint32_t>(std::min<double>(cus * kInsertBlocksPerCU, std::max<double>(cus, blocks)));
+}
 
 uint32_t round_threads(uint32_t t) {
   if (t == 0) t = 256;
@@ -158,6 +170,7 @@ struct gx_explorer {
   int W = 1, WT = 1, BW = 16, mode = kModePlain;
   bool stamp = false, lds = false, per_state = false;
   uint32_t threads = 256, blocks = 1, cus = 1;
+  uint32_t ins_blocks = 1;  // FIND-OR-PUT launch: ins_blocks x kInsertThreads
   uint64_t nbuckets = 0, table_bytes = 0, front_cap = 0, cand_cap = 0;
   DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
   hipStream_t st = nullptr;
@@ -198,7 +211,12 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   hipDeviceProp_t prop;
   GX_TRY(hipGetDeviceProperties(&prop, dev));
   e->cus = static_cast<uint32_t>(prop.multiProcessorCount);
-  e->blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : e->cus * (2048 / e->threads);
+  // default grid: 8192 threads per CU (measured best for the state-per-lane
+  // successor kernel on a 256-CU part; grid-stride loops take any size)
+  e->blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : e->cus * (8192 / e->threads);
+  // random bucket reads are latency bound: a deep grid keeps every SIMD at
+  // its register-limited wave count (sized per level in gx_explorer_run)
+  e->ins_blocks = opt->insert_blocks ? opt->insert_blocks : e->cus * kInsertBlocksPerCU;
   size_t free_b = 0, total_b = 0;
   GX_TRY(hipMemGetInfo(&free_b, &total_b));
   // -q in 32-bit ints (cu:1838-1842); the default is a quarter of free
@@ -304,9 +322,13 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ins_ev, exp_ev;
   double hash_ms = 0, exp_ms = 0;
   Ctrl ch{};
+  double front = 1.0, degree = 16.0;  // known at each host sync
   const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     e->evs.used = 0;
+    const uint32_t ins_blocks = opt->insert_blocks ? opt->insert_blocks
+                                : kiters == 1 ? insert_grid(e->cus, front, degree)
+                                              : e->cus * kInsertBlocksPerCU;
     ins_ev.clear();
     exp_ev.clear();
     for (uint32_t it = 0; it < kiters; it++) {
@@ -316,7 +338,7 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
       GX_TRY(launch_expand(ea, e->mode, W, WT, e->lds, e->blocks, e->threads, st));
       GX_TRY(hipEventRecord(a1, st));
       GX_TRY(hipEventRecord(b0, st));
-      GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, e->blocks, e->threads, st));
+      GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, ins_blocks, kInsertThreads, st));
       GX_TRY(hipEventRecord(b1, st));
       GX_TRY(launch_advance(ia.ctrl, st));
       exp_ev.push_back({a0, a1});
@@ -335,6 +357,8 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
       hash_ms += ms;
     }
     res->hash_launches += ins_ev.size();
+    front = static_cast<double>(ch.front_count[ch.cur]);
+    if (ch.expanded) degree = static_cast<double>(ch.transitions) / static_cast<double>(ch.expanded);
     if (opt->verbosity == 1) printf("%u\n", ch.levels_done);  // cu:2129-2131
     if (ch.stop) break;
   }
@@ -442,7 +466,8 @@ static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8
   GX_TRY(hipEventCreate(&e0));
   GX_TRY(hipEventCreate(&e1));
   GX_TRY(hipEventRecord(e0, t->st));
-  hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, prop.multiProcessorCount * 8, 256, t->st);
+  hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, prop.multiProcessorCount * kInsertBlocksPerCU,
+                                kInsertThreads, t->st);
   GX_TRY(hipEventRecord(e1, t->st));
   GX_TRY(hipMemcpyAsync(&c, t->ctrl.p, sizeof c, hipMemcpyDeviceToHost, t->st));
   GX_TRY(hipStreamSynchronize(t->st));

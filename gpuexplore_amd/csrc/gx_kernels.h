@@ -40,6 +40,10 @@ struct InsertArgs {
   int append;       // append new keys to front[cur ^ 1]
 };
 
+// FIND-OR-PUT launch geometry: 256-thread workgroups, 32 per CU by default
+constexpr uint32_t kInsertThreads = 256;
+constexpr uint32_t kInsertBlocksPerCU = 32;
+
 constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic
 
 // A wave's current chunk of the candidate buffer.  Space is reserved with one

@@ -25,6 +25,7 @@
 #include "../../include/gpuexplore.h"
 #include "gx_batch.h"
 #include "gx_device.h"
+#include "gx_kernels.h"
 #include "gx_model.h"
 
 using namespace gx;
@@ -181,7 +182,8 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   SV_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
   hipError_t ce = hipSuccess;
   for (uint32_t r = 0; r < o.rounds_per_step && ce == hipSuccess; r++)
-    ce = launch_round_w(a, s->W, s->BW, s->o.blocks, s->o.threads, s->st);
+    ce = launch_round_w(a, s->W, s->BW, s->o.blocks, s->o.threads,
+                        static_cast<uint32_t>(prop.multiProcessorCount) * kInsertBlocksPerCU, s->st);
   hipError_t ee = hipStreamEndCapture(s->st, &s->graph);
   SV_TRY(ce);
   SV_TRY(ee);

@@ -87,7 +87,8 @@ typedef struct gx_options {
   uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
   uint64_t max_states;       /* stop after the level that reaches this many states (0 = none) */
   uint32_t expand_kernel;    /* successor kernel: 0 auto, 1 process groups, 2 one state per lane */
-  uint32_t reserved;
+  uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per launch (0 = auto: 32 per CU);
+                                -b/-t set the successor kernel's geometry */
 } gx_options;
 
 void gx_options_default(gx_options *o);
