@@ -159,6 +159,7 @@ SIGNATURES = {
     "gx_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gx_explorer_create": (ctypes.c_int, [_P, ctypes.POINTER(Options), ctypes.POINTER(_P)]),
     "gx_explorer_run": (ctypes.c_int, [_P, ctypes.POINTER(Result)]),
+    "gx_explorer_dump": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "gx_explorer_free": (None, [_P]),
     "gx_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
     "gx_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),
@@ -284,6 +285,10 @@ class Explorer:
         if rc not in (GX_OK, -10):
             raise GxError(rc, "explorer run")
         return d
+
+    def dump(self, path: str) -> None:
+        """gx_explorer_dump: the last run's visited set in the --dump format."""
+        _check(lib().gx_explorer_dump(self.h, path.encode()), "dump")
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value and _lib is not None:

@@ -131,16 +131,20 @@ int main(int argc, char **argv) {
   o.por = por;
   o.cycle_proviso = proviso;
   o.verbosity = verbosity;
-  (void)dump_file;
   gx_result r;
-  rc = gx_explore(m, &o, &r);
+  memset(&r, 0, sizeof r);
+  gx_explorer *ex = nullptr;
+  rc = gx_explorer_create(m, &o, &ex);
+  if (rc == GX_OK) rc = gx_explorer_run(ex, &r);
   if (rc == GX_E_NODEVICE) {  // cu:2000-2003
     fprintf(stderr, "ERROR: No HIP compatible GPU detected!\n");
+    gx_explorer_free(ex);
     gx_model_free(m);
     return 1;
   }
   if (rc != GX_OK && rc != GX_E_CAPACITY) {
     fprintf(stderr, "ERROR: %s\n", gx_strerror(rc));
+    gx_explorer_free(ex);
     gx_model_free(m);
     return 1;
   }
@@ -160,6 +164,19 @@ int main(int argc, char **argv) {
   fprintf(stdout, "nr. of BFS levels: %llu\n", static_cast<unsigned long long>(r.levels));
   if (r.t_explore_s > 0)
     fprintf(stdout, "new states/s: %.0f\n", static_cast<double>(r.inserts) / r.t_explore_s);
+  if (dump_file) {  // cu:2174-2185
+    FILE *t = fopen(dump_file, "w");
+    if (t) {
+      fclose(t);
+      fprintf(stdout, "Dumping state space to file...\n");
+      fflush(stdout);
+      const int drc = gx_explorer_dump(ex, dump_file);
+      if (drc != GX_OK) fprintf(stderr, "ERROR: %s\n", gx_strerror(drc));
+    } else {
+      fprintf(stderr, "Could not open file to dump the state space\n");
+    }
+  }
+  gx_explorer_free(ex);
   gx_model_free(m);
   return 0;  // cu:2187
 }

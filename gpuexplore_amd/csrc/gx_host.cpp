@@ -11,6 +11,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/gpuexplore.h"
@@ -81,6 +82,14 @@ uint32_t insert_grid(uint32_t cus, double front, double degree) {
   const double waves = cand / (16.0 * 8.0) + 1.0;
   const double blocks = std::ceil(waves / (kInsertThreads / 64));
   return static_cast<uint32_t>(std::min<double>(cus * kInsertBlocksPerCU, std::max<double>(cus, blocks)));
+}
+
+// successor grid for a frontier of `front` states: about two 64-state
+// batches per wave, between one workgroup per CU and `max_blocks`
+uint32_t expand_grid(uint32_t cus, uint32_t threads, uint32_t max_blocks, double front) {
+  const double waves = front / (64.0 * 2.0) + 1.0;
+  const double blocks = std::ceil(waves / (threads / 64));
+  return static_cast<uint32_t>(std::min<double>(max_blocks, std::max<double>(cus, blocks)));
 }
 
 uint32_t round_threads(uint32_t t) {
@@ -175,10 +184,71 @@ struct gx_explorer {
   DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
   hipStream_t st = nullptr;
   Events evs;
+  Ctrl last{};       // control block at the end of the last run
+  bool ran = false;
   ~gx_explorer() {
     if (st) (void)hipStreamDestroy(st);
   }
 };
+
+namespace {
+
+// Lists the visited set in the reference's format (cu:544-568, 594-612):
+// per stored state an optional "new: " (not yet expanded), the local states
+// "p0,p1,...", a space, the sv_nints 32-bit words of the packed vector with
+// the flag bits stripped ("%d " each), a newline; then the summary line.
+// States come in table order.  Unexplored = the frontier that the next
+// level would expand.  The table is streamed to the host in slices.
+int print_states(gx_explorer *e, FILE *f, const Ctrl &ch) {
+  const Compiled &c = e->m->comp;
+  const Gpf &g = e->m->gpf;
+  const uint32_t N = g.nr_procs, nw32 = g.sv_nints();
+  const int W = e->W, WT = e->WT, BW = e->BW;
+  const int SPB = BW / WT;
+  std::vector<uint32_t> vec(nw32 ? nw32 : 1);
+  auto words32 = [&](const uint64_t *key) {
+    key_to_words32(c, key, vec.data(), nw32);
+    return std::string(reinterpret_cast<const char *>(vec.data()), nw32 * 4);
+  };
+  // unexplored states: the current frontier
+  std::unordered_set<std::string> fresh;
+  const uint64_t nf = ch.front_count[ch.cur];
+  if (nf) {
+    std::vector<uint64_t> fk(nf * W);
+    const DevBuf &fb = ch.cur ? e->d_front1 : e->d_front0;
+    GX_TRY(hipMemcpyAsync(fk.data(), fb.p, nf * W * 8, hipMemcpyDeviceToHost, e->st));
+    GX_TRY(hipStreamSynchronize(e->st));
+    for (uint64_t i = 0; i < nf; i++) fresh.insert(words32(&fk[i * W]));
+  }
+  const uint64_t slice = 1u << 20;  // buckets per copy
+  std::vector<uint64_t> buf(slice * BW);
+  uint64_t count = 0, newcount = 0;
+  for (uint64_t b0 = 0; b0 < e->nbuckets; b0 += slice) {
+    const uint64_t nb = std::min<uint64_t>(slice, e->nbuckets - b0);
+    GX_TRY(hipMemcpyAsync(buf.data(), e->d_table.as<uint64_t>() + b0 * BW, nb * BW * 8, hipMemcpyDeviceToHost, e->st));
+    GX_TRY(hipStreamSynchronize(e->st));
+    for (uint64_t b = 0; b < nb; b++)
+      for (int s = 0; s < SPB; s++) {
+        const uint64_t *slot = &buf[b * BW + s * WT];
+        if (slot[0] == 0) continue;
+        count++;
+        const std::string w = words32(slot);
+        if (fresh.count(w)) {
+          newcount++;
+          fputs("new: ", f);
+        }
+        for (uint32_t p = 0; p < N; p++) fprintf(f, p + 1 < N ? "%u," : "%u", key_get(c, slot, p));
+        fputc(' ', f);
+        for (uint32_t j = 0; j < nw32; j++) fprintf(f, "%d ", static_cast<int32_t>(vec[j]));
+        fputc('\n', f);
+      }
+  }
+  fprintf(f, "nr. of states in hash table: %llu (%llu unexplored states)\n", static_cast<unsigned long long>(count),
+          static_cast<unsigned long long>(newcount));
+  return GX_OK;
+}
+
+}  // namespace
 
 extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_explorer **out) {
   if (!m || !opt || !out) return GX_E_ARG;
@@ -265,6 +335,17 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
 
 extern "C" void gx_explorer_free(gx_explorer *e) { delete e; }
 
+extern "C" int gx_explorer_dump(gx_explorer *e, const char *path) {
+  if (!e) return GX_E_ARG;
+  if (!e->ran) return GX_E_ARG;
+  FILE *f = path ? fopen(path, "w") : stdout;
+  if (!f) return GX_E_NOFILE;
+  const int rc = print_states(e, f, e->last);
+  if (path) fclose(f);
+  else fflush(stdout);
+  return rc;
+}
+
 extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   if (!e || !res) return GX_E_ARG;
   memset(res, 0, sizeof *res);
@@ -329,13 +410,15 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
     const uint32_t ins_blocks = opt->insert_blocks ? opt->insert_blocks
                                 : kiters == 1 ? insert_grid(e->cus, front, degree)
                                               : e->cus * kInsertBlocksPerCU;
+    const uint32_t exp_blocks = (opt->blocks || kiters != 1) ? e->blocks
+                                : expand_grid(e->cus, e->threads, e->blocks, front);
     ins_ev.clear();
     exp_ev.clear();
     for (uint32_t it = 0; it < kiters; it++) {
       hipEvent_t a0 = e->evs.next(), a1 = e->evs.next(), b0 = e->evs.next(), b1 = e->evs.next();
       if (!a0 || !a1 || !b0 || !b1) return GX_E_HIP;
       GX_TRY(hipEventRecord(a0, st));
-      GX_TRY(launch_expand(ea, e->mode, W, WT, e->lds, e->blocks, e->threads, st));
+      GX_TRY(launch_expand(ea, e->mode, W, WT, e->lds, exp_blocks, e->threads, st));
       GX_TRY(hipEventRecord(a1, st));
       GX_TRY(hipEventRecord(b0, st));
       GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, ins_blocks, kInsertThreads, st));
@@ -360,6 +443,19 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
     front = static_cast<double>(ch.front_count[ch.cur]);
     if (ch.expanded) degree = static_cast<double>(ch.transitions) / static_cast<double>(ch.expanded);
     if (opt->verbosity == 1) printf("%u\n", ch.levels_done);  // cu:2129-2131
+    if (opt->verbosity == 2) {  // cu:2132-2135, 631-651
+      GX_TRY(hipMemsetAsync(e->d_count.p, 0, 8, st));
+      GX_TRY(launch_count(T, WT, BW, e->d_count.as<uint64_t>(), e->cus * 8, st));
+      uint64_t n = 0;
+      GX_TRY(hipMemcpyAsync(&n, e->d_count.p, 8, hipMemcpyDeviceToHost, st));
+      GX_TRY(hipStreamSynchronize(st));
+      printf("nr. of states in hash table: %llu (%llu unexplored states)\n", static_cast<unsigned long long>(n),
+             static_cast<unsigned long long>(ch.front_count[ch.cur]));
+    }
+    if (opt->verbosity == 3) {  // cu:2136-2139
+      const int prc = print_states(e, stdout, ch);
+      if (prc != GX_OK) return prc;
+    }
     if (ch.stop) break;
   }
   const auto t1 = std::chrono::steady_clock::now();
@@ -373,6 +469,8 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   GX_TRY(hipMemcpyAsync(&counted, e->d_count.p, 8, hipMemcpyDeviceToHost, st));
   GX_TRY(hipStreamSynchronize(st));
 
+  e->last = ch;
+  e->ran = true;
   res->states = counted;
   res->transitions = ch.transitions;
   res->levels = ch.level;  // levels completed

@@ -122,6 +122,14 @@ int gx_explore(const gx_model *m, const gx_options *o, gx_result *r);
 typedef struct gx_explorer gx_explorer;
 int gx_explorer_create(const gx_model *m, const gx_options *o, gx_explorer **out);
 int gx_explorer_run(gx_explorer *e, gx_result *r);
+/* Writes the visited set of the last run to `path` (stdout when NULL) in the
+ * reference's --dump / -v 3 format: one line per state, "new: " for states
+ * not yet expanded, the local states "p0,p1,...", then the packed 32-bit
+ * vector words; finally "nr. of states in hash table: S (U unexplored
+ * states)".  Replaces the --dump block cu:2174-2185 (format cu:544-568,
+ * 594-612).  GX_E_NOFILE if the file cannot be opened, GX_E_ARG before a
+ * run. */
+int gx_explorer_dump(gx_explorer *e, const char *path);
 void gx_explorer_free(gx_explorer *e);
 
 const char *gx_strerror(int status);
