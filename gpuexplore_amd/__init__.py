@@ -83,7 +83,7 @@ class BatchOptions(ctypes.Structure):
         ("hash_seed", ctypes.c_uint64),
         ("cand_capacity", ctypes.c_uint64),
         ("use_graph", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("insert_blocks", ctypes.c_uint32),
     ]
 
 

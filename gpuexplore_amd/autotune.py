@@ -1,7 +1,8 @@
 """Autotuner for the batched path.
 
 Sweeps threads per workgroup, rounds captured per graph replay (the
-reference's kernel_iters) and hash-bucket width on a given batch, times
+reference's kernel_iters), hash-bucket width and the FIND-OR-PUT grid on a
+given batch, times
 each setting with gx_explore_batch, and returns the fastest.  Every setting
 must produce identical per-instance counts: a setting that does not is a
 bug, and the sweep raises instead of picking it.
@@ -20,6 +21,7 @@ DEFAULT_SPACE = {
     "threads": [128, 256, 512],
     "levels_per_sync": [1, 4, 16],
     "bucket_bytes": [64, 128],
+    "insert_blocks": [0, 1024, 4096],  # 0: 8 per CU
 }
 
 

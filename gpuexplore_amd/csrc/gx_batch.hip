@@ -164,7 +164,7 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
     bool active = ci < n;
     uint64_t key[W];
 #pragma unroll
-    for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
+    for (int j = 0; j < W; j++) key[j] = active ? __builtin_nontemporal_load(a.cand + ci * W + j) : kFlag;
     active = active && key[0] != 0;
     const uint32_t i = active ? a.cand_inst[ci] : 0u;
     const BatchInst *I = &a.inst[i];
@@ -442,7 +442,8 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   hipDeviceProp_t prop;
   BX_TRY(hipGetDeviceProperties(&prop, dev));
   const uint32_t blocks = o->blocks ? o->blocks : prop.multiProcessorCount * (2048 / threads);
-  const uint32_t ins_blocks = prop.multiProcessorCount * kInsertBlocksPerCU;
+  // rounds of a batch are small and captured once: a modest insert grid
+  const uint32_t ins_blocks = o->insert_blocks ? o->insert_blocks : prop.multiProcessorCount * 8;
   // table partitions proportional to the weights (equal without weights)
   // default: 2^22 ints (16 MiB) per instance
   const uint64_t q_ints = o->q_ints ? o->q_ints : (static_cast<uint64_t>(n) << 22);

@@ -261,6 +261,57 @@ __device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t 
     if (j == j_of1) { e1 = v[j]; c1 = f.cmp[j]; }
   }
   const bool in0 = s_of0 < SPB, in1 = s_of1 < SPB;
+  if constexpr (WT == 2) {
+    // 16-byte slots: lane `sub` holds slot `sub` whole, so the group's match,
+    // half-published and empty masks are three ballots (same protocol as the
+    // general path below)
+    constexpr uint64_t gm = (LPB >= 64) ? ~0ull : ((1ull << LPB) - 1ull);
+    while (ballot(status == kPending)) {
+      if (status == kPending) {
+        if (load) {
+          bucket = T.words + bucket_of(h, probe, T.nbuckets) * BW;
+          if (load == 1) {
+            const ulonglong2 q = *reinterpret_cast<const ulonglong2 *>(bucket + x0);
+            w0 = q.x;
+            w1 = q.y;
+          } else {
+            w0 = load_coherent(bucket + x0);
+            w1 = load_coherent(bucket + x1);
+          }
+        }
+        const bool first = ((w0 ^ e0) & c0) == 0;
+        const uint64_t mm = (ballot(first && ((w1 ^ e1) & c1) == 0) >> group_base) & gm;
+        const uint64_t pm = (ballot(first && w1 == 0) >> group_base) & gm;
+        const uint64_t em = (ballot(w0 == 0) >> group_base) & gm;
+        if (mm) {
+          status = kFound;
+        } else if (pm) {
+          load = 2;
+          if (++spins > kSpinLimit) status = kTimeout;
+        } else if (!em) {
+          load = 1;
+          if (++probe == kHashFunctions) status = kFull;
+        } else {
+          const int owner = __ffsll(static_cast<unsigned long long>(em)) - 1;
+          uint64_t old = 0;
+          if (sub == owner) {
+            old = atomicCAS(reinterpret_cast<unsigned long long *>(bucket + 2 * owner), 0ull,
+                            static_cast<unsigned long long>(v[0]));
+            if (old == 0)
+              store_coherent(bucket + 2 * owner + 1, v[1]);
+            else
+              w0 = old;
+          }
+          old = __shfl(old, group_base + owner, 64);
+          if (old == 0)
+            status = kInserted;
+          else
+            load = 0;
+        }
+      }
+    }
+    return status;
+  }
   while (ballot(status == kPending)) {
     if (status == kPending) {
       if (load) {

@@ -40,9 +40,9 @@ struct InsertArgs {
   int append;       // append new keys to front[cur ^ 1]
 };
 
-// FIND-OR-PUT launch geometry: 256-thread workgroups, 32 per CU by default
+// FIND-OR-PUT launch geometry: 256-thread workgroups, 64 per CU by default
 constexpr uint32_t kInsertThreads = 256;
-constexpr uint32_t kInsertBlocksPerCU = 32;
+constexpr uint32_t kInsertBlocksPerCU = 64;
 
 constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic
 

@@ -475,7 +475,8 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
     bool active = ci < n;
     uint64_t key[W];
 #pragma unroll
-    for (int j = 0; j < W; j++) key[j] = active ? a.cand[ci * W + j] : kFlag;
+    for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
+      key[j] = active ? __builtin_nontemporal_load(a.cand + ci * W + j) : kFlag;
     active = active && key[0] != 0;  // released reservation
     uint64_t v[WT];
     slot_value<W, WT>(key, fmt, v);

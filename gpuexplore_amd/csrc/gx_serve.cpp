@@ -10,7 +10,7 @@
 //   2. replays the captured rounds once (each round expands one BFS level of
 //      every running job: the jobs' work overlaps on the device),
 //   3. retires finished jobs; a job whose partition filled up is queued
-//      again with twice the pages, so the result never depends on the
+//      again with four times the pages, so the result never depends on the
 //      estimate.
 // Visited sets and frontiers of running jobs stay resident between steps.
 #include <hip/hip_runtime.h>
@@ -183,7 +183,7 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   hipError_t ce = hipSuccess;
   for (uint32_t r = 0; r < o.rounds_per_step && ce == hipSuccess; r++)
     ce = launch_round_w(a, s->W, s->BW, s->o.blocks, s->o.threads,
-                        static_cast<uint32_t>(prop.multiProcessorCount) * kInsertBlocksPerCU, s->st);
+                        static_cast<uint32_t>(prop.multiProcessorCount) * 8, s->st);
   hipError_t ee = hipStreamEndCapture(s->st, &s->graph);
   SV_TRY(ce);
   SV_TRY(ee);
@@ -321,7 +321,9 @@ extern "C" int gx_server_step(gx_server *s, gx_server_status *status) {
     for (uint64_t q = 0; q < s->slot_npages[i]; q++) s->page_used[s->slot_page0[i] + q] = 0;
     s->slot_used[i] = 0;
     if ((I.table_full || I.overflow) && j.pages * 2 <= s->npages) {
-      j.pages *= 2;  // restart with a larger partition
+      // restart with a larger partition: 4x (2x at the pool's limit); a
+      // full partition usually means the estimate was off by a large factor
+      j.pages = j.pages * 4 <= s->npages ? j.pages * 4 : j.pages * 2;
       j.restarts++;
       s->waiting.push_front(std::move(j));
       continue;

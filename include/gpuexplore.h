@@ -87,7 +87,7 @@ typedef struct gx_options {
   uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
   uint64_t max_states;       /* stop after the level that reaches this many states (0 = none) */
   uint32_t expand_kernel;    /* successor kernel: 0 auto, 1 process groups, 2 one state per lane */
-  uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per launch (0 = auto: 32 per CU);
+  uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per launch (0 = auto: up to 64 per CU);
                                 -b/-t set the successor kernel's geometry */
 } gx_options;
 
@@ -151,7 +151,7 @@ typedef struct gx_batch_options {
   uint64_t hash_seed;
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint32_t use_graph;        /* 1: replay captured rounds; 0: launch them one by one */
-  uint32_t reserved;
+  uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per round (0 = 8 per CU) */
 } gx_batch_options;
 
 typedef struct gx_batch_stats {
@@ -173,7 +173,7 @@ void gx_batch_options_default(gx_batch_options *o);
  * contiguous run of visited-set pages for the job's estimated footprint are
  * free, replays `rounds_per_step` BFS rounds of all running jobs, and
  * retires finished ones.  A job whose partition fills up is restarted with
- * twice the pages, so results equal gx_explore's whatever the estimate.
+ * four times the pages, so results equal gx_explore's whatever the estimate.
  * Visited sets and frontiers of running jobs stay on the device between
  * steps.  No reference counterpart (the reference runs one model per
  * process). */
