@@ -45,6 +45,9 @@ struct ModelHdr {
   uint32_t off_rules;      // N-bit process masks
   uint32_t off_rowlab;     // nrows x 2 words: mask of the synchronising labels of the row (nlab <= 64)
   uint32_t lab64;          // 1 when every label fits the 64-bit row masks
+  // packed copies for the state-per-lane kernels (16-byte aligned offsets)
+  uint32_t off_rowrec;     // nrows x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi, 0, 0
+  uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
 };
 
 enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u };

@@ -241,6 +241,23 @@ int compile_model(const Gpf &g, Compiled *out) {
   h.off_rule_start = put(rule_start);
   h.off_rules = put(rules);
   h.off_rowlab = put(rowlab);
+  // packed records (uint4 / uint2 reads in the state-per-lane kernels)
+  std::vector<uint32_t> rowrec, synrec;
+  const uint32_t nrec = static_cast<uint32_t>(rowflags.size());
+  rowrec.reserve(8u * nrec);
+  for (uint32_t r = 0; r < nrec; r++) {
+    rowrec.insert(rowrec.end(), {loc_start[r], loc_start[r + 1], syn_start[r], rowflags[r], rowlab[2 * r],
+                                 rowlab[2 * r + 1], 0u, 0u});
+  }
+  for (size_t e = 0; e < syn_tstart.size(); e++) {
+    synrec.push_back(syn_tstart[e]);
+    synrec.push_back(syn_tcnt[e]);
+  }
+  while (B.size() % 4) B.push_back(0);
+  h.off_rowrec = put(rowrec);
+  while (B.size() % 4) B.push_back(0);
+  h.off_synrec = put(synrec);
+  while (B.size() % 4) B.push_back(0);
   h.total_words = static_cast<uint32_t>(B.size());
   out->hdr = h;
   return GX_OK;

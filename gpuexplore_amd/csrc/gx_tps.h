@@ -8,32 +8,64 @@
 // in ascending order with the product of target lists, lowest process
 // varying fastest (cu:951-1130); deadlock per cu:1134-1151; safety per
 // cu:926-935, 1063-1069 checked on the successors that move the last process.
+//
+// Row data comes from the packed 8-word row records (one 16-byte and one
+// 8-byte read per process) and (tstart, tcnt) pairs per (row, label) entry.
+// The enabled sets of up to kTpsLabCache labels are computed in one pass over
+// the processes and handed from the scan to the emission.
 #pragma once
 #include "gx_device.h"
 
 namespace gx {
 
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+constexpr int kTpsLabCache = 4;
 
-__device__ __forceinline__ uint64_t tps_rowlab(const uint32_t *M, const ModelHdr &H, uint32_t row) {
-  return static_cast<uint64_t>(M[H.off_rowlab + 2 * row]) |
-         (static_cast<uint64_t>(M[H.off_rowlab + 2 * row + 1]) << 32);
+// loc_start, loc_end, syn_start, flags of a row
+__device__ __forceinline__ uint4 tps_row(const uint32_t *M, const ModelHdr &H, uint32_t row) {
+  return *reinterpret_cast<const uint4 *>(M + H.off_rowrec + 8u * row);
 }
 
-// index of the (row, label l) entry: l is present in the row's label mask
-__device__ __forceinline__ uint32_t tps_syn_entry(const uint32_t *M, const ModelHdr &H, uint32_t row,
-                                                  uint32_t l) {
-  const uint64_t below = (l >= 64) ? ~0ull : ((1ull << l) - 1ull);
-  return M[H.off_syn_start + row] + static_cast<uint32_t>(__popcll(tps_rowlab(M, H, row) & below));
+// synchronising labels of a row (labels < 64)
+__device__ __forceinline__ uint64_t tps_rowlab(const uint32_t *M, const ModelHdr &H, uint32_t row) {
+  const uint2 v = *reinterpret_cast<const uint2 *>(M + H.off_rowrec + 8u * row + 4u);
+  return static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+}
+
+// (tstart, tcnt) of the (row, l) entry: l is present in the row's label mask
+__device__ __forceinline__ uint2 tps_syn(const uint32_t *M, const ModelHdr &H, uint32_t row, uint32_t l) {
+  const uint64_t below = (1ull << l) - 1ull;
+  const uint32_t e = M[H.off_rowrec + 8u * row + 2u] + static_cast<uint32_t>(__popcll(tps_rowlab(M, H, row) & below));
+  return *reinterpret_cast<const uint2 *>(M + H.off_synrec + 2u * e);
+}
+
+// processes whose current row has an entry for label l
+__device__ __forceinline__ uint32_t tps_enabled(const uint32_t *M, const ModelHdr &H, const uint32_t *rows, int lane,
+                                                int N, uint32_t l) {
+  uint32_t en = 0;
+  for (int p = 0; p < N; p++) {
+    const uint32_t row = rows[p * 64 + lane];
+    if (row != kNoRow && ((tps_rowlab(M, H, row) >> l) & 1ull)) en |= 1u << p;
+  }
+  return en;
 }
 
 struct TpsInfo {
   uint64_t labs;  // synchronising labels present in the state
   uint64_t cnt;   // successors
+  uint32_t en[kTpsLabCache];  // enabled sets of the first labels of labs (when cached)
+  bool cached;    // labs has at most kTpsLabCache labels
   bool dead;      // no local entry and no enabled rule
   bool viol;      // a successor has the last process in local state 1
   bool bad;       // a local state without transition offsets was reached
 };
+
+// enabled set of the i-th label l of labs
+__device__ __forceinline__ uint32_t tps_en_of(const TpsInfo &inf, int i, const uint32_t *M, const ModelHdr &H,
+                                              const uint32_t *rows, int lane, int N, uint32_t l) {
+  if (!inf.cached) return tps_enabled(M, H, rows, lane, N, l);
+  return i == 0 ? inf.en[0] : i == 1 ? inf.en[1] : i == 2 ? inf.en[2] : inf.en[3];
+}
 
 // Pass 1 for the lane's state: rows, successor count, deadlock, safety.
 template <int W>
@@ -42,36 +74,56 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   const int N = static_cast<int>(H.nprocs);
   const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
   const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
-  TpsInfo r{0, 0, false, false, false};
+  TpsInfo r{};
   bool anyloc = false, anyen = false;
   for (int p = 0; p < N; p++) {
     uint32_t row = kNoRow;
+    uint4 ra = make_uint4(0, 0, 0, 0);
     if (has) {
       const uint32_t sp = kget<W>(key, M[H.off_floc + p]);
       if (sp < M[H.off_rowcnt + p]) {
         row = M[H.off_rowbase + p] + sp;
-        if (M[H.off_rowflags + row] & kRowBad) row = kNoRow;
+        ra = tps_row(M, H, row);
+        if (ra.w & kRowBad) row = kNoRow;
       }
       if (row == kNoRow) r.bad = true;
     }
     rows[p * 64 + lane] = row;
     if (row != kNoRow) {
-      const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
-      r.cnt += l1 - l0;
-      anyloc |= (M[H.off_rowflags + row] & kRowHasLocal) != 0;
+      r.cnt += ra.y - ra.x;
+      anyloc |= (ra.w & kRowHasLocal) != 0;
       r.labs |= tps_rowlab(M, H, row);
       if (safety && p == N - 1)
-        for (uint32_t t = l0; t < l1; t++)
+        for (uint32_t t = ra.x; t < ra.y; t++)
           if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) r.viol = true;
     }
   }
-  for (uint64_t L = r.labs; L; L &= L - 1) {
-    const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
-    uint32_t en = 0;
+  // enabled sets of the first labels in one pass over the processes
+  uint32_t lab[kTpsLabCache];
+  {
+    uint64_t L = r.labs;
+#pragma unroll
+    for (int i = 0; i < kTpsLabCache; i++) {
+      lab[i] = L ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1) : 64u;
+      L &= L - 1;
+      r.en[i] = 0;
+    }
+    r.cached = (L == 0);
+  }
+  if (r.cached && r.labs) {
     for (int p = 0; p < N; p++) {
       const uint32_t row = rows[p * 64 + lane];
-      if (row != kNoRow && ((tps_rowlab(M, H, row) >> l) & 1ull)) en |= 1u << p;
+      if (row == kNoRow) continue;
+      const uint64_t rl = tps_rowlab(M, H, row);
+#pragma unroll
+      for (int i = 0; i < kTpsLabCache; i++)
+        if (lab[i] < 64u && ((rl >> lab[i]) & 1ull)) r.en[i] |= 1u << p;
     }
+  }
+  int i = 0;
+  for (uint64_t L = r.labs; L; L &= L - 1, i++) {
+    const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
+    const uint32_t en = tps_en_of(r, i, M, H, rows, lane, N, l);
     const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
     for (uint32_t j = rs; j < re; j++) {
       const uint32_t rule = M[H.off_rules + j];
@@ -80,14 +132,13 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
       uint64_t prod = 1;
       for (uint32_t rr = rule; rr; rr &= rr - 1) {
         const int q = __ffs(rr) - 1;
-        prod *= M[H.off_syn_tcnt + tps_syn_entry(M, H, rows[q * 64 + lane], l)];
+        prod *= tps_syn(M, H, rows[q * 64 + lane], l).y;
       }
       r.cnt += prod;
       if (safety && ((rule >> (N - 1)) & 1u)) {
-        const uint32_t e = tps_syn_entry(M, H, rows[(N - 1) * 64 + lane], l);
-        const uint32_t ts = M[H.off_syn_tstart + e], tc = M[H.off_syn_tcnt + e];
-        for (uint32_t i = 0; i < tc; i++)
-          if ((M[H.off_syn_tgt + ts + i] & fmask_last) == 1u) r.viol = true;
+        const uint2 e = tps_syn(M, H, rows[(N - 1) * 64 + lane], l);
+        for (uint32_t t = 0; t < e.y; t++)
+          if ((M[H.off_syn_tgt + e.x + t] & fmask_last) == 1u) r.viol = true;
       }
     }
   }
@@ -101,14 +152,14 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
 // successor buffer).
 template <int W, typename Emit>
 __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
-                                         const uint32_t *rows, int lane, uint64_t labs, Emit &&emit) {
+                                         const uint32_t *rows, int lane, const TpsInfo &inf, Emit &&emit) {
   const int N = static_cast<int>(H.nprocs);
   for (int p = 0; p < N; p++) {
     const uint32_t row = rows[p * 64 + lane];
     if (row == kNoRow) continue;
     const uint32_t fl = M[H.off_floc + p];
-    const uint32_t l0 = M[H.off_loc_start + row], l1 = M[H.off_loc_start + row + 1];
-    for (uint32_t t = l0; t < l1; t++) {
+    const uint4 ra = tps_row(M, H, row);
+    for (uint32_t t = ra.x; t < ra.y; t++) {
       uint64_t s[W];
 #pragma unroll
       for (int j = 0; j < W; j++) s[j] = key[j];
@@ -116,32 +167,36 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
       emit(s);
     }
   }
-  for (uint64_t L = labs; L; L &= L - 1) {
+  int i = 0;
+  for (uint64_t L = inf.labs; L; L &= L - 1, i++) {
     const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
-    uint32_t en = 0;
-    for (int p = 0; p < N; p++) {
-      const uint32_t row = rows[p * 64 + lane];
-      if (row != kNoRow && ((tps_rowlab(M, H, row) >> l) & 1ull)) en |= 1u << p;
-    }
+    const uint32_t en = tps_en_of(inf, i, M, H, rows, lane, N, l);
     const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
     for (uint32_t j = rs; j < re; j++) {
       const uint32_t rule = M[H.off_rules + j];
       if (rule & ~en) continue;
+      // combination 0 (every member at its first target) with the product
+      uint64_t s[W];
+#pragma unroll
+      for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
       uint32_t prod = 1;
-      for (uint32_t rr = rule; rr; rr &= rr - 1)
-        prod *= M[H.off_syn_tcnt + tps_syn_entry(M, H, rows[(__ffs(rr) - 1) * 64 + lane], l)];
-      for (uint32_t k = 0; k < prod; k++) {
-        uint64_t s[W];
+      for (uint32_t rr = rule; rr; rr &= rr - 1) {
+        const int q = __ffs(rr) - 1;
+        const uint2 e = tps_syn(M, H, rows[q * 64 + lane], l);
+        prod *= e.y;
+        kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x]);
+      }
+      emit(s);
+      for (uint32_t k = 1; k < prod; k++) {  // lowest process varies fastest
 #pragma unroll
         for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
         uint32_t rem = k;
-        for (uint32_t rr = rule; rr; rr &= rr - 1) {  // lowest process varies fastest
+        for (uint32_t rr = rule; rr; rr &= rr - 1) {
           const int q = __ffs(rr) - 1;
-          const uint32_t e = tps_syn_entry(M, H, rows[q * 64 + lane], l);
-          const uint32_t tc = M[H.off_syn_tcnt + e];
-          const uint32_t d = rem % tc;
-          rem /= tc;
-          kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + M[H.off_syn_tstart + e] + d]);
+          const uint2 e = tps_syn(M, H, rows[q * 64 + lane], l);
+          const uint32_t d = rem % e.y;
+          rem /= e.y;
+          kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x + d]);
         }
         emit(s);
       }
