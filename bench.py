@@ -88,7 +88,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sample-states", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane")
+    ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane, 3 fused level")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -199,7 +199,7 @@ def main() -> int:
             "levels": runs[0]["levels"],
             "key_words": info["key_words"],
             "bucket_bytes": args.bucket_bytes,
-            "expand_kernel": ["auto", "process-groups", "state-per-lane"][args.expand_kernel],
+            "expand_kernel": ["auto", "process-groups", "state-per-lane", "fused"][args.expand_kernel],
         },
         "roofline": {
             "bound": "hbm",

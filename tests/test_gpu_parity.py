@@ -92,6 +92,24 @@ def test_small_models_group_kernel(gpu, oracle_mod, model_dir, name, net):
         assert same(a, b), (prop, a, b)
 
 
+@pytest.mark.parametrize("name,net", small_models(), ids=[n for n, _ in small_models()])
+def test_small_models_fused_kernel(gpu, oracle_mod, model_dir, name, net):
+    """plain mode through the fused level kernel (successors and FIND-OR-PUT
+    of a batch by the same wave); cand_capacity 2^15 over 32 workgroups gives
+    slices of 256 entries per wave, so larger batches are split into windows"""
+    p = os.path.join(model_dir, name)
+    net.write(p)
+    for prop in (0, 1, 2):
+        for kw in ({}, {"cand_capacity": 1 << 15, "blocks": 32}):
+            say(name, prop, "fused", kw)
+            a = oracle_mod.explore(p, prop)
+            b = gpu_run(gpu, p, prop, expand_kernel=3, **kw)
+            if b.get("status", 0) == -10:  # a state with more successors than a slice
+                assert kw
+                continue
+            assert same(a, b), (prop, kw, a, b)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_config1(gpu, oracle_mod, model_dir, seed):
     p = os.path.join(model_dir, "c1")
