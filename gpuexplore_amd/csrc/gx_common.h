@@ -50,7 +50,8 @@ struct ModelHdr {
   uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
 };
 
-enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u };
+// kRowMultiSync: a synchronising entry of the row has a target count other than 1
+enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u, kRowMultiSync = 4u };
 
 // Exploration modes of the successor kernel.
 enum ExpandMode : int { kModePlain = 0, kModePor = 1, kModePorProviso = 2 };

@@ -176,6 +176,7 @@ int compile_model(const Gpf &g, Compiled *out) {
       }
       uint64_t lm = 0;
       for (auto &kv : by_label) {  // ascending labels
+        if (kv.second.size() != 1) flags |= kRowMultiSync;  // products other than 1
         if (kv.first < 64) lm |= 1ull << kv.first;
         syn_lab.push_back(kv.first);
         syn_tstart.push_back(static_cast<uint32_t>(syn_tgt.size()));

@@ -12,7 +12,12 @@
 // Row data comes from the packed 8-word row records (one 16-byte and one
 // 8-byte read per process) and (tstart, tcnt) pairs per (row, label) entry.
 // The enabled sets of up to kTpsLabCache labels are computed in one pass over
-// the processes and handed from the scan to the emission.
+// the processes and handed from the scan to the emission, together with the
+// lane's enabled rules of each such label as a bit mask: the 64 lanes of a
+// wave hold different states, and walking every rule of a label would make
+// each lane pay for the rules enabled in any lane.  A rule's product is 1
+// unless a member's row has a synchronising entry with a target count other
+// than 1 (kRowMultiSync), so members are visited only then.
 #pragma once
 #include "gx_device.h"
 
@@ -53,12 +58,18 @@ __device__ __forceinline__ uint32_t tps_enabled(const uint32_t *M, const ModelHd
 struct TpsInfo {
   uint64_t labs;  // synchronising labels present in the state
   uint64_t cnt;   // successors
-  uint32_t en[kTpsLabCache];  // enabled sets of the first labels of labs (when cached)
+  uint32_t en[kTpsLabCache];     // enabled sets of the first labels of labs (when cached)
+  uint64_t rmask[kTpsLabCache];  // enabled rules of those labels, bit k = rule start + k
   bool cached;    // labs has at most kTpsLabCache labels
+  bool rcached;   // ... and each of them has at most 64 rules
   bool dead;      // no local entry and no enabled rule
   bool viol;      // a successor has the last process in local state 1
   bool bad;       // a local state without transition offsets was reached
 };
+
+__device__ __forceinline__ uint64_t tps_rmask_of(const TpsInfo &inf, int i) {
+  return i == 0 ? inf.rmask[0] : i == 1 ? inf.rmask[1] : i == 2 ? inf.rmask[2] : inf.rmask[3];
+}
 
 // enabled set of the i-th label l of labs
 __device__ __forceinline__ uint32_t tps_en_of(const TpsInfo &inf, int i, const uint32_t *M, const ModelHdr &H,
@@ -76,6 +87,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
   TpsInfo r{};
   bool anyloc = false, anyen = false;
+  uint32_t multi = 0;  // processes whose row has a multi-target synchronising entry
   for (int p = 0; p < N; p++) {
     uint32_t row = kNoRow;
     uint4 ra = make_uint4(0, 0, 0, 0);
@@ -92,6 +104,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     if (row != kNoRow) {
       r.cnt += ra.y - ra.x;
       anyloc |= (ra.w & kRowHasLocal) != 0;
+      if (ra.w & kRowMultiSync) multi |= 1u << p;
       r.labs |= tps_rowlab(M, H, row);
       if (safety && p == N - 1)
         for (uint32_t t = ra.x; t < ra.y; t++)
@@ -120,19 +133,25 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
         if (lab[i] < 64u && ((rl >> lab[i]) & 1ull)) r.en[i] |= 1u << p;
     }
   }
+  r.rcached = r.cached;
   int i = 0;
   for (uint64_t L = r.labs; L; L &= L - 1, i++) {
     const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
     const uint32_t en = tps_en_of(r, i, M, H, rows, lane, N, l);
     const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
+    if (re - rs > 64) r.rcached = false;
+    uint64_t rm = 0;
     for (uint32_t j = rs; j < re; j++) {
       const uint32_t rule = M[H.off_rules + j];
       if (rule & ~en) continue;
       anyen = true;
+      if (j - rs < 64) rm |= 1ull << (j - rs);
       uint64_t prod = 1;
-      for (uint32_t rr = rule; rr; rr &= rr - 1) {
-        const int q = __ffs(rr) - 1;
-        prod *= tps_syn(M, H, rows[q * 64 + lane], l).y;
+      if (rule & multi) {
+        for (uint32_t rr = rule; rr; rr &= rr - 1) {
+          const int q = __ffs(rr) - 1;
+          prod *= tps_syn(M, H, rows[q * 64 + lane], l).y;
+        }
       }
       r.cnt += prod;
       if (safety && ((rule >> (N - 1)) & 1u)) {
@@ -141,6 +160,10 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
           if ((M[H.off_syn_tgt + e.x + t] & fmask_last) == 1u) r.viol = true;
       }
     }
+    if (i == 0) r.rmask[0] = rm;
+    if (i == 1) r.rmask[1] = rm;
+    if (i == 2) r.rmask[2] = rm;
+    if (i == 3) r.rmask[3] = rm;
   }
   r.dead = has && !anyloc && !anyen;
   if (!has) r.cnt = 0;
@@ -170,11 +193,17 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
   int i = 0;
   for (uint64_t L = inf.labs; L; L &= L - 1, i++) {
     const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
-    const uint32_t en = tps_en_of(inf, i, M, H, rows, lane, N, l);
     const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
-    for (uint32_t j = rs; j < re; j++) {
+    // the lane's enabled rules: from the scan's mask, else by walking the label's rules
+    const uint32_t en = inf.rcached ? 0u : tps_en_of(inf, i, M, H, rows, lane, N, l);
+    uint64_t rm = inf.rcached ? tps_rmask_of(inf, i) : 0ull;
+    for (uint32_t j = rs; inf.rcached ? rm != 0 : j < re; j++) {
+      if (inf.rcached) {
+        j = rs + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(rm)) - 1);
+        rm &= rm - 1;
+      }
       const uint32_t rule = M[H.off_rules + j];
-      if (rule & ~en) continue;
+      if (!inf.rcached && (rule & ~en)) continue;
       // combination 0 (every member at its first target) with the product
       uint64_t s[W];
 #pragma unroll
@@ -184,8 +213,9 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
         const int q = __ffs(rr) - 1;
         const uint2 e = tps_syn(M, H, rows[q * 64 + lane], l);
         prod *= e.y;
-        kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x]);
+        if (e.y) kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x]);
       }
+      if (prod == 0) continue;  // a member without targets: the rule yields nothing
       emit(s);
       for (uint32_t k = 1; k < prod; k++) {  // lowest process varies fastest
 #pragma unroll

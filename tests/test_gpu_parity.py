@@ -186,3 +186,19 @@ def test_tiny_table_explore_reports_full(gpu, model_dir):
     say("tiny table explore")
     r = gpu_run(gpu, p, 1, q_ints=1024)
     assert r["table_full"] == 1
+
+
+def test_empty_sync_entry(gpu, oracle_mod, model_dir):
+    """a synchronising entry without targets (outside the generator's
+    constraints): no successor from its rules, in every kernel and mode"""
+    from quirk_models import empty_sync_entry_gpf
+    p = os.path.join(model_dir, "quirk_empty")
+    open(p + ".gpf", "w").write(empty_sync_entry_gpf())
+    for prop in (0, 1, 2):
+        for por, prov in MODES:
+            a = oracle_mod.explore(p, prop, por, prov)
+            kernels = (0, 1, 3) if not por else (0,)
+            for ek in kernels:
+                say("quirk empty entry", prop, por, prov, ek)
+                b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
+                assert same(a, b), (prop, por, prov, ek, a, b)

@@ -157,3 +157,19 @@ def test_seed_invariance(oracle_mod, model_dir):
     a = oracle_mod.explore(p, 1, seed=1)
     b = oracle_mod.explore(p, 1, seed=987654321)
     assert (a["states"], a["transitions"], a["levels"]) == (b["states"], b["transitions"], b["levels"])
+
+
+@pytest.mark.parametrize("prop", ["", "deadlock", "safety"])
+@pytest.mark.parametrize("por,proviso", [(False, False), (True, False), (True, True)])
+def test_empty_sync_entry_oracle_vs_naive(oracle_mod, tmp_path, prop, por, proviso):
+    """an entry that keeps its label but has no targets: the rule stays
+    enabled and contributes no successors"""
+    from quirk_models import empty_sync_entry_gpf
+    text = empty_sync_entry_gpf()
+    p = str(tmp_path / "q")
+    open(p + ".gpf", "w").write(text)
+    want = NaiveModel(text).explore({"": "", "deadlock": "d", "safety": "p"}[prop], por=por, proviso=proviso)
+    code = {"": 0, "deadlock": 1, "safety": 2}[prop]
+    got = oracle_mod.explore(p, code, por=por, proviso=proviso)
+    assert (got["states"], got["transitions"], got["levels"], got["violation"]) == \
+        (want["states"], want["transitions"], want["levels"], want["violation"])
