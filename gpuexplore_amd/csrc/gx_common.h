@@ -48,7 +48,12 @@ struct ModelHdr {
   // packed copies for the state-per-lane kernels (16-byte aligned offsets)
   uint32_t off_rowrec;     // nrows x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi, 0, 0
   uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
+  uint32_t off_fdesc;      // per process x 8 words: field descriptor (key word, shift, value
+                           // mask, 63 - shift, 64-bit in-word mask, 64-bit mask in the next word)
+  uint32_t tps_begin;      // first word of the section the state-per-lane kernels read
+                           // (rowbase .. fdesc, 4-word aligned, up to total_words)
 };
+
 
 // kRowMultiSync: a synchronising entry of the row has a target count other than 1
 enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u, kRowMultiSync = 4u };

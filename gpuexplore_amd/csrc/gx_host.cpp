@@ -326,11 +326,12 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   // whose labels fit the 64-bit row masks, else the process-group kernel
   const size_t lds_limit = 64 * 1024;
   const size_t model_b = ((H.total_words + 3u) & ~3u) * 4u;
-  const size_t tps_b = expand_tps_scratch_bytes(e->threads, H.nprocs);
+  const size_t sec_b = ((H.total_words - H.tps_begin + 3u) & ~3u) * 4u;  // state-per-lane section
+  const size_t tps_b = expand_tps_scratch_bytes(e->threads, H.nprocs, H.nrows < 0xFFFFu);
   e->per_state = e->mode == kModePlain && H.lab64 && opt->expand_kernel != 1 && tps_b <= lds_limit;
   if (opt->expand_kernel == 2 && !e->per_state) return GX_E_ARG;
   if (e->per_state)
-    e->lds = model_b + tps_b <= lds_limit;
+    e->lds = sec_b + tps_b <= lds_limit;
   else
     e->lds = model_b + expand_scratch_bytes(e->threads) <= 48 * 1024;
   // fused level (successors + FIND-OR-PUT in one kernel, 256 threads): each
@@ -351,7 +352,7 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
         GX_TRY(e->d_cand.alloc(e->cand_cap * W * 8));
       }
     }
-    e->lds = model_b + fused_scratch_bytes(256, H.nprocs, W) <= lds_limit;
+    e->lds = sec_b + fused_scratch_bytes(256, H.nprocs, W) <= lds_limit;
     if (!e->lds && fused_scratch_bytes(256, H.nprocs, W) > lds_limit) return GX_E_LIMIT;
   }
   *out = guard.release();

@@ -23,9 +23,11 @@ struct ExpandArgs {
   int per_state;     // plain mode: one state per lane (k_expand_tps)
 };
 
-// LDS bytes k_expand_tps needs besides the staged model.
-inline size_t expand_tps_scratch_bytes(uint32_t threads, uint32_t nprocs) {
-  return static_cast<size_t>(threads / 64) * nprocs * 64 * 4;
+// LDS bytes k_expand_tps needs besides the staged model: per wave a
+// [process][lane] table of row indices, 2 bytes each when the model has fewer
+// than 0xFFFF rows.
+inline size_t expand_tps_scratch_bytes(uint32_t threads, uint32_t nprocs, bool rows16) {
+  return static_cast<size_t>(threads / 64) * nprocs * 64 * (rows16 ? 2 : 4) + 16;
 }
 
 // Fused level (plain method, one state per lane): every wave expands its

@@ -347,18 +347,20 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
 // state) a lane expanding a whole state is cheaper than a group of N lanes:
 // 64 states per wave instead of floor(64/N), and no cross-lane traffic
 // except the wave's output reservation (gx_tps.h has the per-state code).
-template <int W, bool LDS>
+template <int W, bool LDS, typename RowT>
 __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   extern __shared__ uint32_t smem[];
-  const ModelHdr &H = a.hdr;
+  // with LDS, only the state-per-lane section of the model is staged
+  const ModelHdr H = LDS ? tps_rebased(a.hdr) : a.hdr;
   Ctrl *C = a.ctrl;
+  const uint32_t sec = a.hdr.total_words - a.hdr.tps_begin;
   if (LDS) {
-    for (uint32_t i = threadIdx.x; i < H.total_words; i += blockDim.x) smem[i] = a.model[i];
+    for (uint32_t i = threadIdx.x; i < sec; i += blockDim.x) smem[i] = a.model[a.hdr.tps_begin + i];
   }
   const uint32_t *M = LDS ? smem : a.model;
-  const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
+  const uint32_t xoff = LDS ? ((sec + 3u) & ~3u) : 0u;
   const int N = static_cast<int>(H.nprocs);
-  uint32_t *rows = smem + xoff + static_cast<uint32_t>(threadIdx.x >> 6) * (N * 64u);
+  RowT *rows = reinterpret_cast<RowT *>(smem + xoff) + static_cast<uint32_t>(threadIdx.x >> 6) * (N * 64u);
   __syncthreads();
   if (C->stop) return;
   const uint32_t cur = C->cur;
@@ -441,13 +443,14 @@ size_t fused_scratch_bytes(uint32_t threads, uint32_t nprocs, int key_words) {
 template <int W, int BW, bool LDS>
 __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
   extern __shared__ uint32_t smem[];
-  const ModelHdr &H = a.hdr;
+  const ModelHdr H = LDS ? tps_rebased(a.hdr) : a.hdr;
   Ctrl *C = a.ctrl;
+  const uint32_t sec = a.hdr.total_words - a.hdr.tps_begin;
   if (LDS) {
-    for (uint32_t i = threadIdx.x; i < H.total_words; i += blockDim.x) smem[i] = a.model[i];
+    for (uint32_t i = threadIdx.x; i < sec; i += blockDim.x) smem[i] = a.model[a.hdr.tps_begin + i];
   }
   const uint32_t *M = LDS ? smem : a.model;
-  const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
+  const uint32_t xoff = LDS ? ((sec + 3u) & ~3u) : 0u;
   const int N = static_cast<int>(H.nprocs);
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t nwb = blockDim.x >> 6;
@@ -573,7 +576,8 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
 
 template <int W>
 static hipError_t fused_bw(const FusedArgs &a, int BW, bool lds, uint32_t blocks, uint32_t threads, hipStream_t s) {
-  const size_t shm = fused_scratch_bytes(threads, a.hdr.nprocs, W) + (lds ? (a.hdr.total_words + 3u) / 4u * 16u : 0u);
+  const size_t shm = fused_scratch_bytes(threads, a.hdr.nprocs, W) +
+                     (lds ? (a.hdr.total_words - a.hdr.tps_begin + 3u) / 4u * 16u : 0u);
 #define GX_FB(BWk)                                                                                        \
   if (lds) hipLaunchKernelGGL((k_fused<W, BWk, true>), dim3(blocks), dim3(threads), shm, s, a);           \
   else hipLaunchKernelGGL((k_fused<W, BWk, false>), dim3(blocks), dim3(threads), shm, s, a);              \
@@ -752,11 +756,17 @@ static hipError_t expand_w(const ExpandArgs &a, int mode, bool lds, uint32_t blo
                            uint32_t threads, hipStream_t s) {
   const size_t model_b = lds ? ((a.hdr.total_words + 3u) & ~3u) * 4u : 0u;
   if (mode == kModePlain && a.per_state) {
-    const size_t shm = model_b + static_cast<size_t>(threads / 64) * a.hdr.nprocs * 64 * 4;
-    if (lds)
-      hipLaunchKernelGGL((k_expand_tps<W, true>), dim3(blocks), dim3(threads), shm, s, a);
+    const size_t sec_b = lds ? ((a.hdr.total_words - a.hdr.tps_begin + 3u) & ~3u) * 4u : 0u;
+    const bool r16 = a.hdr.nrows < 0xFFFFu;
+    const size_t shm = sec_b + expand_tps_scratch_bytes(threads, a.hdr.nprocs, r16);
+    if (lds && r16)
+      hipLaunchKernelGGL((k_expand_tps<W, true, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
+    else if (lds)
+      hipLaunchKernelGGL((k_expand_tps<W, true, uint32_t>), dim3(blocks), dim3(threads), shm, s, a);
+    else if (r16)
+      hipLaunchKernelGGL((k_expand_tps<W, false, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
     else
-      hipLaunchKernelGGL((k_expand_tps<W, false>), dim3(blocks), dim3(threads), shm, s, a);
+      hipLaunchKernelGGL((k_expand_tps<W, false, uint32_t>), dim3(blocks), dim3(threads), shm, s, a);
     return hipGetLastError();
   }
   const size_t shm = model_b + expand_scratch_bytes(threads);

@@ -228,20 +228,25 @@ int compile_model(const Gpf &g, Compiled *out) {
     B.insert(B.end(), v.begin(), v.end());
     return off;
   };
+  // arrays only the process-group kernel reads come first; everything the
+  // state-per-lane kernels read forms one section from tps_begin to the end,
+  // which is all they stage in LDS
   h.off_floc = put(floc);
-  h.off_rowbase = put(rowbase);
-  h.off_rowcnt = put(rowcnt);
   h.off_loc_start = put(loc_start);
   h.off_syn_start = put(syn_start);
   h.off_rowflags = put(rowflags);
-  h.off_loc_tgt = put(loc_tgt);
   h.off_syn_lab = put(syn_lab);
   h.off_syn_tstart = put(syn_tstart);
   h.off_syn_tcnt = put(syn_tcnt);
+  h.off_rowlab = put(rowlab);
+  while (B.size() % 4) B.push_back(0);
+  h.tps_begin = static_cast<uint32_t>(B.size());
+  h.off_rowbase = put(rowbase);
+  h.off_rowcnt = put(rowcnt);
+  h.off_loc_tgt = put(loc_tgt);
   h.off_syn_tgt = put(syn_tgt);
   h.off_rule_start = put(rule_start);
   h.off_rules = put(rules);
-  h.off_rowlab = put(rowlab);
   // packed records (uint4 / uint2 reads in the state-per-lane kernels)
   std::vector<uint32_t> rowrec, synrec;
   const uint32_t nrec = static_cast<uint32_t>(rowflags.size());
@@ -258,6 +263,19 @@ int compile_model(const Gpf &g, Compiled *out) {
   h.off_rowrec = put(rowrec);
   while (B.size() % 4) B.push_back(0);
   h.off_synrec = put(synrec);
+  std::vector<uint32_t> fdesc;
+  for (uint32_t p = 0; p < N; p++) {
+    const FieldLoc &f = out->floc[p];
+    const uint64_t vmask = f.width >= 32 ? 0xFFFFFFFFull : ((1ull << f.width) - 1ull);
+    const uint64_t mlo = (vmask << f.offset) & kPayload;
+    const bool straddle = f.offset + f.width > 63;
+    const uint64_t mhi = straddle ? ((1ull << (f.offset + f.width - 63)) - 1ull) : 0ull;
+    fdesc.insert(fdesc.end(), {f.chunk, f.offset, static_cast<uint32_t>(vmask), 63u - f.offset,
+                               static_cast<uint32_t>(mlo), static_cast<uint32_t>(mlo >> 32),
+                               static_cast<uint32_t>(mhi), static_cast<uint32_t>(mhi >> 32)});
+  }
+  while (B.size() % 4) B.push_back(0);
+  h.off_fdesc = put(fdesc);
   while (B.size() % 4) B.push_back(0);
   h.total_words = static_cast<uint32_t>(B.size());
   out->hdr = h;

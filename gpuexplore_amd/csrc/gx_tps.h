@@ -23,7 +23,34 @@
 
 namespace gx {
 
+// The header with the state-per-lane section's offsets rebased onto a copy
+// of words [tps_begin, total_words) (the LDS-staged section).
+__device__ __forceinline__ ModelHdr tps_rebased(const ModelHdr &h) {
+  ModelHdr r = h;
+  const uint32_t b = h.tps_begin;
+  r.off_rowbase -= b;
+  r.off_rowcnt -= b;
+  r.off_loc_tgt -= b;
+  r.off_syn_tgt -= b;
+  r.off_rule_start -= b;
+  r.off_rules -= b;
+  r.off_rowrec -= b;
+  r.off_synrec -= b;
+  r.off_fdesc -= b;
+  return r;
+}
+
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+
+// rows[p * 64 + lane] holds row indices in RowT (uint16_t when the model has
+// fewer than 0xFFFF rows, halving the LDS they take); the all-ones value of
+// RowT stands for kNoRow
+template <typename RowT>
+__device__ __forceinline__ uint32_t row_at(const RowT *rows, int i) {
+  const uint32_t v = rows[i];
+  if constexpr (sizeof(RowT) == 2) return v == 0xFFFFu ? kNoRow : v;
+  return v;
+}
 constexpr int kTpsLabCache = 4;
 
 // loc_start, loc_end, syn_start, flags of a row
@@ -45,11 +72,12 @@ __device__ __forceinline__ uint2 tps_syn(const uint32_t *M, const ModelHdr &H, u
 }
 
 // processes whose current row has an entry for label l
-__device__ __forceinline__ uint32_t tps_enabled(const uint32_t *M, const ModelHdr &H, const uint32_t *rows, int lane,
+template <typename RowT>
+__device__ __forceinline__ uint32_t tps_enabled(const uint32_t *M, const ModelHdr &H, const RowT *rows, int lane,
                                                 int N, uint32_t l) {
   uint32_t en = 0;
   for (int p = 0; p < N; p++) {
-    const uint32_t row = rows[p * 64 + lane];
+    const uint32_t row = row_at(rows, p * 64 + lane);
     if (row != kNoRow && ((tps_rowlab(M, H, row) >> l) & 1ull)) en |= 1u << p;
   }
   return en;
@@ -72,19 +100,19 @@ __device__ __forceinline__ uint64_t tps_rmask_of(const TpsInfo &inf, int i) {
 }
 
 // enabled set of the i-th label l of labs
+template <typename RowT>
 __device__ __forceinline__ uint32_t tps_en_of(const TpsInfo &inf, int i, const uint32_t *M, const ModelHdr &H,
-                                              const uint32_t *rows, int lane, int N, uint32_t l) {
+                                              const RowT *rows, int lane, int N, uint32_t l) {
   if (!inf.cached) return tps_enabled(M, H, rows, lane, N, l);
   return i == 0 ? inf.en[0] : i == 1 ? inf.en[1] : i == 2 ? inf.en[2] : inf.en[3];
 }
 
 // Pass 1 for the lane's state: rows, successor count, deadlock, safety.
-template <int W>
+template <int W, typename RowT>
 __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
-                                            bool has, uint32_t *rows, int lane, bool safety) {
+                                            bool has, RowT *rows, int lane, bool safety) {
   const int N = static_cast<int>(H.nprocs);
-  const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
-  const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
+  const uint32_t fmask_last = field_desc(M, H.off_fdesc, N - 1).vmask;
   TpsInfo r{};
   bool anyloc = false, anyen = false;
   uint32_t multi = 0;  // processes whose row has a multi-target synchronising entry
@@ -92,7 +120,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     uint32_t row = kNoRow;
     uint4 ra = make_uint4(0, 0, 0, 0);
     if (has) {
-      const uint32_t sp = kget<W>(key, M[H.off_floc + p]);
+      const uint32_t sp = kget_d<W>(key, field_desc(M, H.off_fdesc, p));
       if (sp < M[H.off_rowcnt + p]) {
         row = M[H.off_rowbase + p] + sp;
         ra = tps_row(M, H, row);
@@ -100,7 +128,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
       }
       if (row == kNoRow) r.bad = true;
     }
-    rows[p * 64 + lane] = row;
+    rows[p * 64 + lane] = static_cast<RowT>(row);
     if (row != kNoRow) {
       r.cnt += ra.y - ra.x;
       anyloc |= (ra.w & kRowHasLocal) != 0;
@@ -125,7 +153,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   }
   if (r.cached && r.labs) {
     for (int p = 0; p < N; p++) {
-      const uint32_t row = rows[p * 64 + lane];
+      const uint32_t row = row_at(rows, p * 64 + lane);
       if (row == kNoRow) continue;
       const uint64_t rl = tps_rowlab(M, H, row);
 #pragma unroll
@@ -150,12 +178,12 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
       if (rule & multi) {
         for (uint32_t rr = rule; rr; rr &= rr - 1) {
           const int q = __ffs(rr) - 1;
-          prod *= tps_syn(M, H, rows[q * 64 + lane], l).y;
+          prod *= tps_syn(M, H, row_at(rows, q * 64 + lane), l).y;
         }
       }
       r.cnt += prod;
       if (safety && ((rule >> (N - 1)) & 1u)) {
-        const uint2 e = tps_syn(M, H, rows[(N - 1) * 64 + lane], l);
+        const uint2 e = tps_syn(M, H, row_at(rows, (N - 1) * 64 + lane), l);
         for (uint32_t t = 0; t < e.y; t++)
           if ((M[H.off_syn_tgt + e.x + t] & fmask_last) == 1u) r.viol = true;
       }
@@ -173,20 +201,20 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
 // Pass 2: calls emit(s) for every successor s of the lane's state, in the
 // order counted by tps_scan.  Products are below 2^32 (they fit the
 // successor buffer).
-template <int W, typename Emit>
+template <int W, typename RowT, typename Emit>
 __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
-                                         const uint32_t *rows, int lane, const TpsInfo &inf, Emit &&emit) {
+                                         const RowT *rows, int lane, const TpsInfo &inf, Emit &&emit) {
   const int N = static_cast<int>(H.nprocs);
   for (int p = 0; p < N; p++) {
-    const uint32_t row = rows[p * 64 + lane];
+    const uint32_t row = row_at(rows, p * 64 + lane);
     if (row == kNoRow) continue;
-    const uint32_t fl = M[H.off_floc + p];
+    const FieldDesc fd = field_desc(M, H.off_fdesc, p);
     const uint4 ra = tps_row(M, H, row);
     for (uint32_t t = ra.x; t < ra.y; t++) {
       uint64_t s[W];
 #pragma unroll
       for (int j = 0; j < W; j++) s[j] = key[j];
-      kset<W>(s, fl, M[H.off_loc_tgt + t]);
+      kset_d<W>(s, fd, M[H.off_loc_tgt + t]);
       emit(s);
     }
   }
@@ -211,9 +239,9 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
       uint32_t prod = 1;
       for (uint32_t rr = rule; rr; rr &= rr - 1) {
         const int q = __ffs(rr) - 1;
-        const uint2 e = tps_syn(M, H, rows[q * 64 + lane], l);
+        const uint2 e = tps_syn(M, H, row_at(rows, q * 64 + lane), l);
         prod *= e.y;
-        if (e.y) kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x]);
+        if (e.y) kset_d<W>(s, field_desc(M, H.off_fdesc, q), M[H.off_syn_tgt + e.x]);
       }
       if (prod == 0) continue;  // a member without targets: the rule yields nothing
       emit(s);
@@ -223,10 +251,10 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
         uint32_t rem = k;
         for (uint32_t rr = rule; rr; rr &= rr - 1) {
           const int q = __ffs(rr) - 1;
-          const uint2 e = tps_syn(M, H, rows[q * 64 + lane], l);
+          const uint2 e = tps_syn(M, H, row_at(rows, q * 64 + lane), l);
           const uint32_t d = rem % e.y;
           rem /= e.y;
-          kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + e.x + d]);
+          kset_d<W>(s, field_desc(M, H.off_fdesc, q), M[H.off_syn_tgt + e.x + d]);
         }
         emit(s);
       }
