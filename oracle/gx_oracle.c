@@ -291,7 +291,7 @@ static int enumerate(const orc_model *m, const uint32_t *s, const sview *v, uint
                      succ_fn fn, void *ctx, sinfo *info) {
   uint32_t W = m->words, N = m->nr_procs;
   uint32_t t[ORC_MAX_WORDS];
-  static uint32_t tg[ORC_MAX_PROCS][MAXT];
+  static __thread uint32_t tg[ORC_MAX_PROCS][MAXT]; /* per thread: orc_explore_mt */
   uint32_t tn[ORC_MAX_PROCS];
   memset(info, 0, sizeof *info);
   for (uint32_t p = 0; p < N; p++) info->cluster0[p] = 1ull << p;
@@ -691,5 +691,192 @@ out:
   vset_free(&set);
   free(cur.v);
   free(nxt.v);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Multi-threaded plain BFS (the CPU baseline at several cores): the same
+ * level-synchronous semantics as orc_explore without POR.  The visited set
+ * is cut into shards, each an open-addressing vset behind a mutex; threads
+ * take frontier chunks from an atomic counter and keep private next-level
+ * lists, joined between levels.  Counts equal orc_explore's. */
+#include <pthread.h>
+#include <stdatomic.h>
+
+#define MT_SHARDS 4096u
+
+typedef struct {
+  vset set;
+  pthread_mutex_t mu;
+} mt_shard;
+
+typedef struct {
+  const orc_model *m;
+  const orc_options *o;
+  mt_shard *shards;
+  const uint32_t *cur;
+  uint64_t ncur;
+  _Atomic uint64_t next_chunk;
+  uint32_t level;
+} mt_shared;
+
+typedef struct {
+  mt_shared *sh;
+  vlist next;
+  uint64_t transitions;
+  int violation, err;
+} mt_worker;
+
+static uint32_t mt_shard_of(const mt_shared *sh, const uint32_t *k) {
+  uint64_t h = sh->o->hash_seed ^ 0x2545f4914f6cdd1dull;
+  for (uint32_t i = 0; i < sh->m->words; i++) h = mix64(h ^ k[i]) + 0x9e37u * i;
+  return (uint32_t)(h >> 52) & (MT_SHARDS - 1u);
+}
+
+static int mt_on_succ(void *c, const uint32_t *t, int kind, uint32_t who, uint64_t rule) {
+  (void)kind;
+  (void)who;
+  (void)rule;
+  mt_worker *w = (mt_worker *)c;
+  mt_shared *sh = w->sh;
+  w->transitions++;
+  if (sh->o->property == ORC_PROP_SAFETY && is_violating(sh->m, t)) w->violation = 1;
+  mt_shard *s = &sh->shards[mt_shard_of(sh, t)];
+  pthread_mutex_lock(&s->mu);
+  int rc = vset_put(&s->set, t, sh->level + 1);
+  pthread_mutex_unlock(&s->mu);
+  if (rc < 0) return rc;
+  if (rc == 1) return vlist_push(&w->next, t);
+  return ORC_OK;
+}
+
+static void *mt_run(void *arg) {
+  mt_worker *w = (mt_worker *)arg;
+  mt_shared *sh = w->sh;
+  const uint32_t W = sh->m->words;
+  for (;;) {
+    uint64_t c0 = atomic_fetch_add(&sh->next_chunk, 1024u);
+    if (c0 >= sh->ncur) break;
+    uint64_t c1 = c0 + 1024u < sh->ncur ? c0 + 1024u : sh->ncur;
+    for (uint64_t i = c0; i < c1; i++) {
+      const uint32_t *s = sh->cur + i * W;
+      sview v;
+      sinfo info;
+      int rc = view_state(sh->m, s, &v);
+      if (!rc) rc = enumerate(sh->m, s, &v, ~0ull, mt_on_succ, w, &info);
+      if (rc) {
+        w->err = rc;
+        return NULL;
+      }
+      if (sh->o->property == ORC_PROP_DEADLOCK && !(info.any_local_entry || info.any_enabled_rule))
+        w->violation = 1; /* cu:1134-1151 */
+    }
+  }
+  return NULL;
+}
+
+int orc_explore_mt(const orc_model *m, const orc_options *o, int nthreads, orc_result *res) {
+  if (o->por || nthreads < 1 || nthreads > 256) return ORC_ERR_ARG;
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  memset(res, 0, sizeof *res);
+  const uint32_t W = m->words;
+  int rc = ORC_OK;
+  mt_shared sh;
+  memset(&sh, 0, sizeof sh);
+  sh.m = m;
+  sh.o = o;
+  sh.shards = (mt_shard *)calloc(MT_SHARDS, sizeof(mt_shard));
+  mt_worker *ws = (mt_worker *)calloc((size_t)nthreads, sizeof(mt_worker));
+  pthread_t *tids = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  vlist cur = {NULL, 0, 0, W};
+  if (!sh.shards || !ws || !tids) {
+    rc = ORC_ERR_NOMEM;
+    goto out;
+  }
+  for (uint32_t i = 0; i < MT_SHARDS; i++) {
+    rc = vset_init(&sh.shards[i].set, W, 64, o->hash_seed);
+    if (rc) goto out;
+    pthread_mutex_init(&sh.shards[i].mu, NULL);
+  }
+  uint32_t s0[ORC_MAX_WORDS] = {0}; /* cu:673-676 */
+  vset_put(&sh.shards[mt_shard_of(&sh, s0)].set, s0, 0);
+  vlist_push(&cur, s0);
+  for (int t = 0; t < nthreads; t++) {
+    ws[t].sh = &sh;
+    ws[t].next.W = W;
+  }
+  uint64_t states = 1, transitions = 0;
+  int violation = 0;
+  uint32_t level = 0;
+  while (cur.n > 0) {
+    sh.cur = cur.v;
+    sh.ncur = cur.n;
+    sh.level = level;
+    atomic_store(&sh.next_chunk, 0);
+    for (int t = 0; t < nthreads; t++) {
+      ws[t].next.n = 0;
+      if (pthread_create(&tids[t], NULL, mt_run, &ws[t])) {
+        rc = ORC_ERR_NOMEM;
+        for (int u = 0; u < t; u++) pthread_join(tids[u], NULL);
+        goto out;
+      }
+    }
+    uint64_t nnext = 0;
+    for (int t = 0; t < nthreads; t++) {
+      pthread_join(tids[t], NULL);
+      if (ws[t].err) rc = ws[t].err;
+      nnext += ws[t].next.n;
+    }
+    if (rc) goto out;
+    /* next frontier: the threads' lists, concatenated */
+    vlist nxt = {NULL, 0, 0, W};
+    if (nnext) {
+      nxt.v = (uint32_t *)malloc(nnext * W * sizeof(uint32_t));
+      if (!nxt.v) {
+        rc = ORC_ERR_NOMEM;
+        goto out;
+      }
+      nxt.cap = nnext;
+      for (int t = 0; t < nthreads; t++) {
+        memcpy(nxt.v + nxt.n * W, ws[t].next.v, ws[t].next.n * W * sizeof(uint32_t));
+        nxt.n += ws[t].next.n;
+      }
+    }
+    transitions = 0;
+    for (int t = 0; t < nthreads; t++) {
+      transitions += ws[t].transitions;
+      violation |= ws[t].violation;
+    }
+    states += nnext;
+    free(cur.v);
+    cur = nxt;
+    level++;
+    res->levels = level;
+    if (violation) break; /* level-granular early stop */
+    if (o->max_states && states >= o->max_states) {
+      res->truncated = 1;
+      break;
+    }
+  }
+  res->states = states;
+  res->transitions = transitions;
+  res->full_successors = transitions;
+  res->violation = violation;
+out:
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  res->seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+  if (sh.shards) {
+    for (uint32_t i = 0; i < MT_SHARDS; i++) {
+      vset_free(&sh.shards[i].set);
+      pthread_mutex_destroy(&sh.shards[i].mu);
+    }
+  }
+  if (ws)
+    for (int t = 0; t < nthreads; t++) free(ws[t].next.v);
+  free(sh.shards);
+  free(ws);
+  free(tids);
+  free(cur.v);
   return rc;
 }

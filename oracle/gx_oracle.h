@@ -74,7 +74,8 @@ enum {
   ORC_ERR_FORMAT = -2,
   ORC_ERR_RANGE = -3,
   ORC_ERR_NOMEM = -4,
-  ORC_ERR_LIMIT = -5
+  ORC_ERR_LIMIT = -5,
+  ORC_ERR_ARG = -6
 };
 
 typedef struct orc_options {
@@ -113,6 +114,10 @@ long orc_successors(const orc_model *m, const uint32_t *state, uint32_t *out,
 uint64_t orc_por_cluster(const orc_model *m, const uint32_t *state);
 
 int orc_explore(const orc_model *m, const orc_options *opt, orc_result *res);
+
+/* orc_explore on `nthreads` threads for the plain method (no POR):
+ * identical counts, sharded visited set; ORC_ERR_ARG under POR. */
+int orc_explore_mt(const orc_model *m, const orc_options *opt, int nthreads, orc_result *res);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,9 @@ def lib():
         L.orc_free.argtypes = [ctypes.POINTER(_Model)]
         L.orc_explore.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Opts), ctypes.POINTER(_Res)]
         L.orc_explore.restype = ctypes.c_int
+        L.orc_explore_mt.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(_Opts), ctypes.c_int,
+                                     ctypes.POINTER(_Res)]
+        L.orc_explore_mt.restype = ctypes.c_int
         L.orc_successors.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(ctypes.c_uint32),
                                      ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t,
                                      ctypes.POINTER(ctypes.c_int)]
@@ -177,7 +180,22 @@ class OracleModel:
                     full=r.full_successors, violation=bool(r.violation),
                     truncated=bool(r.truncated), seconds=r.seconds)
 
+    def explore_mt(self, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:
+        """orc_explore_mt: the plain method on `threads` threads."""
+        o = _Opts(prop, 0, 0, max_states, seed)
+        r = _Res()
+        rc = lib().orc_explore_mt(self.ptr, ctypes.byref(o), threads, ctypes.byref(r))
+        if rc != 0:
+            raise OSError(f"orc_explore_mt failed: {rc}")
+        return dict(states=r.states, transitions=r.transitions, levels=r.levels,
+                    full=r.full_successors, violation=bool(r.violation),
+                    truncated=bool(r.truncated), seconds=r.seconds)
+
 
 def explore(path_noext: str, prop: int = PROP_NONE, por: bool = False, proviso: bool = False,
             max_states: int = 0, seed: int = 0) -> dict:
     return OracleModel(path_noext, por).explore(prop, por, proviso, max_states, seed)
+
+
+def explore_mt(path_noext: str, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:
+    return OracleModel(path_noext, False).explore_mt(threads, prop, max_states, seed)

@@ -173,3 +173,20 @@ def test_empty_sync_entry_oracle_vs_naive(oracle_mod, tmp_path, prop, por, provi
     got = oracle_mod.explore(p, code, por=por, proviso=proviso)
     assert (got["states"], got["transitions"], got["levels"], got["violation"]) == \
         (want["states"], want["transitions"], want["levels"], want["violation"])
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_multithreaded_oracle_equals_sequential(oracle_mod, model_dir, threads):
+    """orc_explore_mt (the multi-core CPU baseline) gives orc_explore's counts"""
+    for name, net in small_models()[:8] + [("c1", gen.config1(2))]:
+        p = os.path.join(model_dir, name)
+        net.write(p)
+        for prop in (0, 1, 2):
+            a = oracle_mod.explore(p, prop)
+            b = oracle_mod.explore_mt(p, threads, prop)
+            assert (a["states"], a["transitions"], a["levels"], a["violation"]) == \
+                (b["states"], b["transitions"], b["levels"], b["violation"]), (name, prop, a, b)
+    a = oracle_mod.explore(p, 0, max_states=5000)
+    b = oracle_mod.explore_mt(p, threads, 0, max_states=5000)
+    assert (a["states"], a["transitions"], a["levels"], a["truncated"]) == \
+        (b["states"], b["transitions"], b["levels"], b["truncated"])
