@@ -202,3 +202,19 @@ def test_empty_sync_entry(gpu, oracle_mod, model_dir):
                 say("quirk empty entry", prop, por, prov, ek)
                 b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
                 assert same(a, b), (prop, por, prov, ek, a, b)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_converted_exp_networks(gpu, oracle_mod, model_dir, seed):
+    """networks converted from EXP synchronisation vectors over .aut processes"""
+    from test_exp2gpf import random_network
+    from gpuexplore_amd import exp2gpf
+    vectors, auts = random_network(seed)
+    p = os.path.join(model_dir, f"exp{seed}")
+    exp2gpf.network_from(vectors, auts).write(p)
+    for prop in (0, 1):
+        for por, prov in MODES:
+            say("exp network", seed, prop, por, prov)
+            a = oracle_mod.explore(p, prop, por, prov)
+            b = gpu_run(gpu, p, prop, por, prov)
+            assert same(a, b), (seed, prop, por, prov, a, b)
