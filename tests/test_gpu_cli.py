@@ -165,3 +165,30 @@ def test_dump_unwritable_path(gpu, tmp_path):
     r = run(gpu, base, "--dump", str(tmp_path / "no" / "such" / "dir" / "f"))
     assert r.returncode == 0
     assert "Could not open file to dump the state space" in r.stderr
+
+
+def final_count(stdout: str) -> int:
+    for ln in reversed(stdout.splitlines()):
+        m = re.match(r"^nr\. of states in hash table: (\d+)$", ln)
+        if m:
+            return int(m.group(1))
+    raise AssertionError("no count line")
+
+
+@pytest.mark.parametrize("por", [False, True])
+def test_liveness_model_with_p(gpu, tmp_path, oracle_mod, por):
+    """gpf line 1 = 1 with -p: the property becomes LIVENESS, which no kernel
+    checks (cu:1888-1898, 2158-2160), so the run is plain reachability with
+    the cycle proviso that -p switches on (cu:1857-1861)"""
+    net = gen.config1(3)
+    net.liveness = 1
+    base = str(tmp_path / "live")
+    net.write(base)
+    args = [base, "-p"] + (["--por"] if por else [])
+    say("liveness", args)
+    r = run(gpu, *args)
+    assert r.returncode == 0, r.stderr
+    assert "Property to check is a liveness property" in r.stdout
+    assert "violation detected" not in r.stdout
+    want = oracle_mod.explore(base, 0, por=por, proviso=por)
+    assert final_count(r.stdout) == want["states"]
