@@ -218,3 +218,23 @@ def test_converted_exp_networks(gpu, oracle_mod, model_dir, seed):
             a = oracle_mod.explore(p, prop, por, prov)
             b = gpu_run(gpu, p, prop, por, prov)
             assert same(a, b), (seed, prop, por, prov, a, b)
+
+
+@pytest.mark.parametrize("name", [n for n, _ in __import__("edge_models").edge_models()])
+def test_edge_models(gpu, oracle_mod, model_dir, name):
+    """3-5 key words, 32 processes, more than 64 labels, in every mode and
+    with every plain-mode kernel that accepts the model"""
+    from edge_models import edge_models
+    p = os.path.join(model_dir, name)
+    dict(edge_models())[name].write(p)
+    for prop in (0, 1, 2):
+        for por, prov in MODES:
+            a = oracle_mod.explore(p, prop, por, prov)
+            for ek in ((0, 1, 3) if not por else (0,)):
+                say("edge", name, prop, por, prov, ek)
+                try:
+                    b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
+                except gpu.GxError as e:
+                    assert ek == 3 and e.status == -7  # GX_E_ARG: no state-per-lane kernel for > 64 labels
+                    continue
+                assert same(a, b), (name, prop, por, prov, ek, a, b)

@@ -190,3 +190,20 @@ def test_multithreaded_oracle_equals_sequential(oracle_mod, model_dir, threads):
     b = oracle_mod.explore_mt(p, threads, 0, max_states=5000)
     assert (a["states"], a["transitions"], a["levels"], a["truncated"]) == \
         (b["states"], b["transitions"], b["levels"], b["truncated"])
+
+
+@pytest.mark.parametrize("name", [n for n, _ in __import__("edge_models").edge_models()])
+def test_edge_models_oracle_vs_naive(oracle_mod, model_dir, name):
+    """3-5 key words, 32 processes, more than 64 labels"""
+    from edge_models import edge_models
+    net = dict(edge_models())[name]
+    p = write(net, model_dir, name)
+    nm = NaiveModel(open(p + ".gpf").read())
+    for prop in ("", "d", "p"):
+        for por, prov in ((False, False), (True, False), (True, True)):
+            if name == "wide5" and (prop, por) != ("", False):
+                continue  # the largest one once: the naive enumerator is slow
+            a = nm.explore(prop, por, prov)
+            b = oracle_mod.explore(p, PROPS[prop], por, prov)
+            for k in ("states", "transitions", "levels", "violation"):
+                assert a[k] == b[k], (prop, por, prov, k, a, b)
