@@ -24,6 +24,7 @@
 #include "gx_model.h"
 #include "gx_tps.h"
 #include "gx_batch.h"
+#include "gx_devbuf.h"
 
 namespace gx {
 
@@ -370,13 +371,6 @@ using namespace gx;
 
 namespace {
 
-struct Buf {
-  void *p = nullptr;
-  ~Buf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t b) { return hipMalloc(&p, b ? b : 8); }
-};
 
 int map_err(hipError_t e) {
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GX_E_NOMEM;
@@ -470,7 +464,7 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   const uint64_t max_chunks = front_total / 64 + static_cast<uint64_t>(n) + 1;
 
   const auto t_init0 = std::chrono::steady_clock::now();
-  Buf d_inst, d_ctrl, d_models, d_f0, d_f1, d_cand, d_cinst, d_def0, d_def1, d_table;
+  DevBuf d_inst, d_ctrl, d_models, d_f0, d_f1, d_cand, d_cinst, d_def0, d_def1, d_table;
   BX_TRY(d_inst.alloc(sizeof(BatchInst) * n));
   BX_TRY(d_ctrl.alloc(sizeof(BatchCtrl)));
   BX_TRY(d_models.alloc(mbuf.size() * 4));

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/gpuexplore.h"
+#include "gx_devbuf.h"
 #include "gx_kernels.h"
 #include "gx_model.h"
 
@@ -41,21 +42,6 @@ int device_count() {
 }
 
 // RAII device buffer
-struct DevBuf {
-  void *p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t bytes) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    return hipMalloc(&p, bytes ? bytes : 8);
-  }
-  template <typename T>
-  T *as() const {
-    return static_cast<T *>(p);
-  }
-};
 
 struct Events {
   std::vector<hipEvent_t> ev;

@@ -24,6 +24,7 @@
 
 #include "../../include/gpuexplore.h"
 #include "gx_batch.h"
+#include "gx_devbuf.h"
 #include "gx_device.h"
 #include "gx_kernels.h"
 #include "gx_model.h"
@@ -47,17 +48,6 @@ int map_err(hipError_t e) {
     if (e_ != hipSuccess) return map_err(e_); \
   } while (0)
 
-struct DBuf {
-  void *p = nullptr;
-  ~DBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t b) { return hipMalloc(&p, b ? b : 8); }
-  template <typename T>
-  T *as() const {
-    return static_cast<T *>(p);
-  }
-};
 
 struct Job {
   uint64_t id;
@@ -75,7 +65,7 @@ struct gx_server {
   gx_server_options o{};
   int W = 2, BW = 8;
   uint64_t spb = 4, page_buckets = 0, page_entries = 0, npages = 0, model_words = 0;
-  DBuf inst, ctrl, models, f0, f1, cand, cinst, def0, def1, table;
+  DevBuf inst, ctrl, models, f0, f1, cand, cinst, def0, def1, table;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

@@ -81,7 +81,8 @@ typedef struct gx_options {
   int32_t property;          /* GX_PROP_* */
   int32_t por;               /* --por */
   int32_t cycle_proviso;     /* --cycle-proviso */
-  int32_t verbosity;         /* -v (only level 0/1 affect this call: progress counters) */
+  int32_t verbosity;         /* -v: 1 prints each synchronised level, 2 the state counts, 3 every
+                                stored state (cu:2128-2140), on stdout */
   uint64_t hash_seed;        /* seeds the bucket hash; counts do not depend on it */
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint64_t frontier_capacity;/* frontier entries per level buffer (0 = auto) */
