@@ -94,7 +94,7 @@ extern "C" void gx_server_options_default(gx_server_options *o) {
   o->key_words = 2;
   o->max_procs = 16;
   o->threads = 512;
-  o->rounds_per_step = 4;
+  o->rounds_per_step = 16;  // a 10 s stream at 20 jobs/s: p50 0.74 ms (4: 0.98 ms, 1: 1.55 ms)
   o->table_bytes = 16ull << 30;
   o->page_bytes = 4ull << 20;
   o->model_words = 1u << 15;

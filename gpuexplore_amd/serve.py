@@ -173,7 +173,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seconds", type=float, default=120.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slots", type=int, default=256)
-    ap.add_argument("--rounds-per-step", type=int, default=4)
+    ap.add_argument("--rounds-per-step", type=int, default=16)
     ap.add_argument("--table-gib", type=float, default=16.0)
     args = ap.parse_args(argv)
     kw = {"slots": args.slots, "rounds_per_step": args.rounds_per_step,

@@ -186,7 +186,7 @@ typedef struct gx_server_options {
   uint32_t max_procs;        /* largest N accepted */
   uint32_t threads;          /* threads per workgroup */
   uint32_t blocks;           /* workgroups per launch (0 = auto) */
-  uint32_t rounds_per_step;  /* BFS rounds per step (one graph replay) */
+  uint32_t rounds_per_step;  /* BFS rounds per step (one graph replay; default 16) */
   uint64_t table_bytes;      /* visited-set pool */
   uint64_t page_bytes;       /* allocation granule of the pool */
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
