@@ -160,8 +160,15 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
     wave_sync();
     staged = 0;
   };
-  for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
+  // runs of consecutive iterations per wave, as in k_insert
+  const uint64_t RUN = CPW * insert_run(n, nw * CPW);
+  for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
     const uint64_t ci = c0 + cg;
+    c0 += CPW;
+    if (c0 == r0 + RUN) {
+      r0 += nw * RUN;
+      c0 = r0;
+    }
     bool active = ci < n;
     uint64_t key[W];
 #pragma unroll

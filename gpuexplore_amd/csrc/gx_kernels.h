@@ -70,6 +70,21 @@ constexpr uint32_t kInsertBlocksPerCU = 64;
 
 constexpr uint64_t kChunk = 1024;  // candidate entries reserved per atomic
 
+// Iterations in a row that one FIND-OR-PUT wave takes from the candidate
+// buffer (`stride` = candidates of one iteration of all waves).  A pass that
+// gives every wave fewer than kInsertSpread iterations is latency bound and
+// spreads over the whole grid; a larger one goes in runs of kInsertRun, even
+// if that leaves waves idle: fewer streams keep a run's recent buckets in
+// the caches until its duplicate lookups arrive (a ring of 20 16-state
+// processes, 295 M states, and 64 models of 4-14 processes measured best at
+// 8 / 64; runs of 64 to 256 are alike).
+constexpr uint64_t kInsertRun = 64;
+constexpr uint64_t kInsertSpread = 8;
+__device__ __forceinline__ uint64_t insert_run(uint64_t n, uint64_t stride) {
+  const uint64_t iters = (n + stride - 1) / stride;
+  return iters < 1 ? 1 : (iters < kInsertSpread ? iters : kInsertRun);
+}
+
 // A wave's current chunk of the candidate buffer.  Space is reserved with one
 // atomicAdd per chunk, sized for the wave's remaining batches (at most kChunk
 // entries unless one batch needs more); unused chunk tails are released by

@@ -649,8 +649,19 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
     wave_sync();
     staged = 0;
   };
-  for (uint64_t c0 = gw * CPW; c0 < n; c0 += nw * CPW) {
+  // Each wave walks runs of consecutive iterations.  Successors of one
+  // state, and of states found next to each other, are adjacent in the
+  // candidate buffer, so the run keeps them on one wave: its new keys reach
+  // the next frontier in candidate order (siblings stay together), and the
+  // duplicates that commuting moves produce are looked up close in time.
+  const uint64_t RUN = CPW * insert_run(n, nw * CPW);
+  for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
     const uint64_t ci = c0 + cg;
+    c0 += CPW;
+    if (c0 == r0 + RUN) {
+      r0 += nw * RUN;
+      c0 = r0;
+    }
     bool active = ci < n;
     uint64_t key[W];
 #pragma unroll
