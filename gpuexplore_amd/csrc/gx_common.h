@@ -50,8 +50,13 @@ struct ModelHdr {
   uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
   uint32_t off_fdesc;      // per process x 8 words: field descriptor (key word, shift, value
                            // mask, 63 - shift, 64-bit in-word mask, 64-bit mask in the next word)
+  uint32_t off_locx;       // per loc_tgt entry: the target placed in its field (key_words
+                           // 64-bit words as lo, hi pairs; zero outside the field)
+  uint32_t off_synx;       // per syn_tgt entry: the same for synchronising targets
+  uint32_t off_fclr;       // per process: its field bits (key-shaped)
+  uint32_t off_rclr;       // per rule: the field bits of its members (key-shaped)
   uint32_t tps_begin;      // first word of the section the state-per-lane kernels read
-                           // (rowbase .. fdesc, 4-word aligned, up to total_words)
+                           // (rowbase .. rclr, 4-word aligned, up to total_words)
 };
 
 

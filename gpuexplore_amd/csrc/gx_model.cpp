@@ -276,6 +276,61 @@ int compile_model(const Gpf &g, Compiled *out) {
   }
   while (B.size() % 4) B.push_back(0);
   h.off_fdesc = put(fdesc);
+  // key-shaped forms (W 64-bit words as lo, hi pairs) for successor
+  // emission: a target value already placed in its process's field bits,
+  // and the field bits of a process and of a rule's members to clear first
+  const uint32_t W = h.key_words;
+  auto field_bits = [&](uint32_t p, std::vector<uint64_t> &k) {
+    const FieldLoc &f = out->floc[p];
+    if (f.width == 0) return;
+    const uint64_t m = (f.width >= 32) ? 0xFFFFFFFFull : ((1ull << f.width) - 1);
+    k[f.chunk] |= (m << f.offset) & kPayload;
+    if (f.offset + f.width > 63) k[f.chunk + 1] |= (1ull << (f.offset + f.width - 63)) - 1;
+  };
+  auto put_key = [&](std::vector<uint32_t> &dst, const std::vector<uint64_t> &k) {
+    for (uint32_t j = 0; j < W; j++) {
+      dst.push_back(static_cast<uint32_t>(k[j]));
+      dst.push_back(static_cast<uint32_t>(k[j] >> 32));
+    }
+  };
+  auto placed = [&](uint32_t p, uint32_t v) {
+    std::vector<uint64_t> k(W, 0);
+    key_set(*out, k.data(), p, v);
+    return k;
+  };
+  std::vector<uint32_t> locx, synx, fclr, rclr;
+  // loc_tgt / syn_tgt entries belong to the process of their row
+  std::vector<uint32_t> row_proc(nrec, 0);
+  for (uint32_t p = 0; p < N; p++)
+    for (uint32_t r = rowbase[p]; r < rowbase[p] + rowcnt[p]; r++) row_proc[r] = p;
+  std::vector<uint32_t> syn_proc(syn_tstart.size(), 0);
+  for (uint32_t r = 0; r < nrec; r++) {
+    for (uint32_t t = loc_start[r]; t < loc_start[r + 1]; t++) put_key(locx, placed(row_proc[r], loc_tgt[t]));
+    for (uint32_t e = syn_start[r]; e < syn_start[r + 1]; e++) syn_proc[e] = row_proc[r];
+  }
+  std::vector<uint32_t> tgt_proc(syn_tgt.size(), 0);
+  for (size_t e = 0; e < syn_tstart.size(); e++)
+    for (uint32_t t = syn_tstart[e]; t < syn_tstart[e] + syn_tcnt[e]; t++) tgt_proc[t] = syn_proc[e];
+  for (size_t t = 0; t < syn_tgt.size(); t++) put_key(synx, placed(tgt_proc[t], syn_tgt[t]));
+  for (uint32_t p = 0; p < N; p++) {
+    std::vector<uint64_t> k(W, 0);
+    field_bits(p, k);
+    put_key(fclr, k);
+  }
+  for (uint32_t r : rules) {
+    std::vector<uint64_t> k(W, 0);
+    for (uint32_t p = 0; p < N; p++)
+      if ((r >> p) & 1u) field_bits(p, k);
+    put_key(rclr, k);
+  }
+  while (B.size() % 4) B.push_back(0);
+  h.off_locx = put(locx);
+  while (B.size() % 4) B.push_back(0);
+  h.off_synx = put(synx);
+  while (B.size() % 4) B.push_back(0);
+  h.off_fclr = put(fclr);
+  while (B.size() % 4) B.push_back(0);
+  h.off_rclr = put(rclr);
   while (B.size() % 4) B.push_back(0);
   h.total_words = static_cast<uint32_t>(B.size());
   out->hdr = h;

@@ -37,6 +37,10 @@ __device__ __forceinline__ ModelHdr tps_rebased(const ModelHdr &h) {
   r.off_rowrec -= b;
   r.off_synrec -= b;
   r.off_fdesc -= b;
+  r.off_locx -= b;
+  r.off_synx -= b;
+  r.off_fclr -= b;
+  r.off_rclr -= b;
   return r;
 }
 
@@ -198,23 +202,59 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   return r;
 }
 
+// The i-th key-shaped record at `off` as W key words.  Records hold the
+// model's own key_words `kw` (lo, hi word pairs); a batch may run the model
+// with wider keys (kw < W), whose upper words are then zero.  16-byte reads
+// when kw == W is even: the records then start 16-byte aligned.
+template <int W>
+__device__ __forceinline__ void tps_keyrec(const uint32_t *M, uint32_t off, uint32_t i, uint32_t kw,
+                                           uint64_t (&x)[W]) {
+  const uint32_t *q = M + off + 2u * kw * i;
+  if constexpr (W % 2 == 0) {
+    if (kw == static_cast<uint32_t>(W)) {
+#pragma unroll
+      for (int j = 0; j < W; j += 2) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(q + 2 * j);
+        x[j] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+        x[j + 1] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    x[j] = 0;
+    if (static_cast<uint32_t>(j) < kw) {
+      const uint2 v = *reinterpret_cast<const uint2 *>(q + 2 * j);
+      x[j] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+    }
+  }
+}
+
 // Pass 2: calls emit(s) for every successor s of the lane's state, in the
-// order counted by tps_scan.  Products are below 2^32 (they fit the
-// successor buffer).
+// order counted by tps_scan.  A successor is the key with the moving
+// processes' fields cleared (a process's or a rule's key-shaped field mask)
+// OR the targets' key-shaped records, which the model holds already placed in
+// their fields.  Products are below 2^32 (they fit the successor buffer).
 template <int W, typename RowT, typename Emit>
 __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
                                          const RowT *rows, int lane, const TpsInfo &inf, Emit &&emit) {
   const int N = static_cast<int>(H.nprocs);
+  const uint32_t kw = H.key_words;
   for (int p = 0; p < N; p++) {
     const uint32_t row = row_at(rows, p * 64 + lane);
     if (row == kNoRow) continue;
-    const FieldDesc fd = field_desc(M, H.off_fdesc, p);
     const uint4 ra = tps_row(M, H, row);
+    if (ra.x == ra.y) continue;
+    uint64_t base[W];
+    tps_keyrec<W>(M, H.off_fclr, static_cast<uint32_t>(p), kw, base);
+#pragma unroll
+    for (int j = 0; j < W; j++) base[j] = key[j] & ~base[j];
     for (uint32_t t = ra.x; t < ra.y; t++) {
       uint64_t s[W];
+      tps_keyrec<W>(M, H.off_locx, t, kw, s);
 #pragma unroll
-      for (int j = 0; j < W; j++) s[j] = key[j];
-      kset_d<W>(s, fd, M[H.off_loc_tgt + t]);
+      for (int j = 0; j < W; j++) s[j] |= base[j];
       emit(s);
     }
   }
@@ -232,29 +272,41 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
       }
       const uint32_t rule = M[H.off_rules + j];
       if (!inf.rcached && (rule & ~en)) continue;
+      uint64_t base[W];
+      tps_keyrec<W>(M, H.off_rclr, j, kw, base);
+#pragma unroll
+      for (int jj = 0; jj < W; jj++) base[jj] = key[jj] & ~base[jj];
       // combination 0 (every member at its first target) with the product
       uint64_t s[W];
 #pragma unroll
-      for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
+      for (int jj = 0; jj < W; jj++) s[jj] = base[jj];
       uint32_t prod = 1;
       for (uint32_t rr = rule; rr; rr &= rr - 1) {
         const int q = __ffs(rr) - 1;
         const uint2 e = tps_syn(M, H, row_at(rows, q * 64 + lane), l);
         prod *= e.y;
-        if (e.y) kset_d<W>(s, field_desc(M, H.off_fdesc, q), M[H.off_syn_tgt + e.x]);
+        if (e.y) {
+          uint64_t x[W];
+          tps_keyrec<W>(M, H.off_synx, e.x, kw, x);
+#pragma unroll
+          for (int jj = 0; jj < W; jj++) s[jj] |= x[jj];
+        }
       }
       if (prod == 0) continue;  // a member without targets: the rule yields nothing
       emit(s);
       for (uint32_t k = 1; k < prod; k++) {  // lowest process varies fastest
 #pragma unroll
-        for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
+        for (int jj = 0; jj < W; jj++) s[jj] = base[jj];
         uint32_t rem = k;
         for (uint32_t rr = rule; rr; rr &= rr - 1) {
           const int q = __ffs(rr) - 1;
           const uint2 e = tps_syn(M, H, row_at(rows, q * 64 + lane), l);
           const uint32_t d = rem % e.y;
           rem /= e.y;
-          kset_d<W>(s, field_desc(M, H.off_fdesc, q), M[H.off_syn_tgt + e.x + d]);
+          uint64_t x[W];
+          tps_keyrec<W>(M, H.off_synx, e.x + d, kw, x);
+#pragma unroll
+          for (int jj = 0; jj < W; jj++) s[jj] |= x[jj];
         }
         emit(s);
       }
