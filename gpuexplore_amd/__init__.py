@@ -22,6 +22,7 @@ ERRORS = {
     -7: "ARG", -8: "LIMIT", -9: "STATE", -10: "CAPACITY",
 }
 PROP_NONE, PROP_DEADLOCK, PROP_SAFETY, PROP_LIVENESS = 0, 1, 2, 3
+POR_CAMPLE, POR_AMPLE, POR_STUBBORN = 0, 1, 2  # gx_options.por_method
 
 
 class GxError(RuntimeError):
@@ -47,6 +48,7 @@ class Options(ctypes.Structure):
         ("max_states", ctypes.c_uint64),
         ("expand_kernel", ctypes.c_uint32),
         ("insert_blocks", ctypes.c_uint32),
+        ("por_method", ctypes.c_uint32),
     ]
 
 

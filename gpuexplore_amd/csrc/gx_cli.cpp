@@ -22,6 +22,8 @@ static const char *kHelp =
     "  -p                 Check a safety property (should be embedded in the model)\n"
     "  --por              Apply partial-order reduction\n"
     "  --cycle-proviso    Apply the cycle proviso during partial-order reduction\n"
+    "  --por-method NAME  Partial-order reduction by cample (default), ample or stubborn sets;\n"
+    "                     implies --por\n"
     "  -k NUM             Run NUM BFS levels per host synchronisation (default 1)\n"
     "  -b NUM             Run the kernels on NUM workgroups (default: fill the GPU)\n"
     "  -t NUM             Use NUM threads per workgroup, a multiple of 64 (default 256)\n"
@@ -89,6 +91,22 @@ int main(int argc, char **argv) {
     } else if (!strcmp(a, "--cycle-proviso")) {
       if (property == GX_PROP_NONE) proviso = 1;
       i += 1;
+    } else if (!strcmp(a, "--por-method") && has_val) {
+      // one binary for the reference's per-method branches (README.md lines 24-26)
+      const char *v = argv[i + 1];
+      if (!strcmp(v, "cample")) {
+        o.por_method = GX_POR_CAMPLE;
+      } else if (!strcmp(v, "ample")) {
+        o.por_method = GX_POR_AMPLE;
+      } else if (!strcmp(v, "stubborn")) {
+        o.por_method = GX_POR_STUBBORN;
+      } else {
+        fprintf(stderr, "ERROR: unknown partial-order reduction method %s\n", v);
+        fputs(kHelp, stdout);
+        return 1;
+      }
+      por = 1;
+      i += 2;
     } else if (!strcmp(a, "--dump") && has_val) {
       dump_file = argv[i + 1];
       i += 2;

@@ -246,6 +246,7 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   const ModelHdr &H = m->comp.hdr;
   const int W = static_cast<int>(H.key_words);
   const bool proviso = opt->por && opt->cycle_proviso;
+  if (opt->por_method > GX_POR_STUBBORN) return GX_E_ARG;
   const uint32_t K = H.state_bits;
   const int WT = proviso ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
   if (WT > kMaxKeyWords || W > kMaxKeyWords) return GX_E_LIMIT;
@@ -392,6 +393,7 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   ea.bucket_words = BW;
   ea.table = T;
   ea.per_state = e->per_state ? 1 : 0;
+  ea.por_method = static_cast<int>(opt->por_method);
   InsertArgs ia{};
   ia.ctrl = e->d_ctrl.as<Ctrl>();
   ia.cand = e->d_cand.as<uint64_t>();

@@ -21,6 +21,7 @@ struct ExpandArgs {
   int bucket_words;  // FIND under the cycle proviso
   Table table;
   int per_state;     // plain mode: one state per lane (k_expand_tps)
+  int por_method;    // GX_POR_* under POR (cample, ample, stubborn)
 };
 
 // LDS bytes k_expand_tps needs besides the staged model: per wave a

@@ -9,6 +9,7 @@
 //              2109-2127)
 // All sizes are read from the device control block, so levels can be
 // enqueued back to back without host round trips.
+#include "../../include/gpuexplore.h"
 #include "gx_kernels.h"
 #include "gx_tps.h"
 
@@ -157,6 +158,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
     // ---------------- pass 1: enabledness, safety, (POR) clusters
     bool any_enabled = false;
     uint32_t cluster0 = in_group ? (1u << p) : 0u;
+    uint32_t stub = cluster0;  // stubborn-set relation d(p)
     bool in_en = false;
     uint32_t rnew = 0;  // enabled rules with a new successor (proviso)
     bool local_new = false;
@@ -172,6 +174,10 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
               const uint32_t r = M[H.off_rules + j];
               if ((r >> p) & 1u) {
                 cluster0 |= r;
+                // the whole rule when every member has the label, else its
+                // lowest member without it (the moves that can enable it)
+                const uint32_t lack = r & ~static_cast<uint32_t>(en);
+                stub |= lack ? (lack & (~lack + 1u)) : r;
                 if ((static_cast<uint64_t>(r) & ~en) == 0 && rule_product(r) > 0) in_en = true;
               }
             }
@@ -236,6 +242,55 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
         const uint32_t acc = group_or(rnew, rot);
         prov = local_new || ((acc >> p) & 1u);
       }
+      if (a.por_method == GX_POR_AMPLE) {
+        // the local moves of one process none of whose current labels has a
+        // rule containing it (cluster0 = {p}), with a (new) local
+        // successor; the fewest local successors win, the lowest on ties
+        const bool cand = in_group && cluster0 == (1u << p) &&
+                          (MODE == kModePorProviso ? local_new : (l1 > l0));
+        xf[lane] = cand ? (l1 - l0) : 0xFFFFFFFFu;
+        wave_sync();
+        if (in_group) {
+          uint32_t best = 0xFFFFFFFFu;
+          for (int q = 0; q < N; q++) {
+            const uint32_t nq = xf[base + q];
+            if (nq < best) {
+              best = nq;
+              csel = 1u << q;
+            }
+          }
+        }
+        wave_sync();
+      } else if (a.por_method == GX_POR_STUBBORN) {
+        // closure of {p} under d; valid when it meets a process with an
+        // enabled (under the proviso: new) successor; the smallest valid
+        // set below N processes wins, the lowest p on ties
+        const uint32_t target = static_cast<uint32_t>((ballot(in_group && prov) >> base) & nmask);
+        xc[lane] = stub;
+        wave_sync();
+        uint32_t K = in_group ? (1u << p) : 0u, done = 0;
+        if (in_group) {
+          while (K & ~done) {
+            const int i = __ffs(K & ~done) - 1;
+            done |= 1u << i;
+            K |= xc[base + i];
+          }
+        }
+        xf[lane] = (K & target) ? K : 0u;
+        wave_sync();
+        if (in_group) {
+          int best = N;
+          for (int q = 0; q < N; q++) {
+            const uint32_t fq = xf[base + q];
+            const int pc = __popc(fq);
+            if (fq && pc < best) {
+              best = pc;
+              csel = fq;
+            }
+          }
+        }
+        wave_sync();
+      } else {
       const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
       xc[lane] = c;
       wave_sync();
@@ -264,6 +319,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
         }
       }
       wave_sync();
+      }
     }
 
     // ---------------- pass 2: count (ample) successors

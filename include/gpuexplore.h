@@ -42,6 +42,25 @@ extern "C" {
 #define GX_PROP_SAFETY 2
 #define GX_PROP_LIVENESS 3
 
+/* Partial-order reduction methods for gx_options.por_method.  The reference
+ * keeps one method per branch (README.md lines 24-26: ample-por, cample-por,
+ * stubborn-por); this snapshot's source holds cample (cu:1340-1675).  AMPLE
+ * and STUBBORN have no source here: they select among the same successors
+ * (local moves of a process set C and the enabled rules inside C) by the
+ * published approaches:
+ *   CAMPLE    process clusters (cu:1616-1672);
+ *   AMPLE     the local moves of one process whose current labels have no
+ *             rule containing it (the fewest local successors, lowest index);
+ *   STUBBORN  closure of {p} under d(p) = p + each rule of p's labels that
+ *             every member has + the lowest member lacking the label of each
+ *             other rule (a necessary enabling set); the smallest closure that
+ *             meets a process with an enabled successor.
+ * With the cycle proviso, AMPLE and STUBBORN require a new successor in the
+ * selected set and otherwise expand fully. */
+#define GX_POR_CAMPLE 0
+#define GX_POR_AMPLE 1
+#define GX_POR_STUBBORN 2
+
 typedef struct gx_model gx_model;
 
 /* Parse <path_noext>.gpf (cu:1814-1815, 1885-1983).  `por` adds the extra
@@ -90,6 +109,7 @@ typedef struct gx_options {
   uint32_t expand_kernel;    /* successor kernel: 0 auto, 1 process groups, 2 one state per lane */
   uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per launch (0 = auto: up to 64 per CU);
                                 -b/-t set the successor kernel's geometry */
+  uint32_t por_method;       /* GX_POR_* when por is set (0 = cample, the reference's method) */
 } gx_options;
 
 void gx_options_default(gx_options *o);

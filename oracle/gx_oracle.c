@@ -281,6 +281,8 @@ typedef struct {
   uint64_t cluster0[ORC_MAX_PROCS];   /* cu:1347, 1616-1624 */
   uint64_t local_succ;    /* processes with >= 1 local successor */
   uint64_t in_enabled;    /* processes in some enabled rule with >= 1 successor */
+  uint32_t nloc[ORC_MAX_PROCS];       /* local successors per process */
+  uint64_t stub[ORC_MAX_PROCS];       /* stubborn-set relation d(p), see select_stubborn */
 } sinfo;
 
 /* Enumerate successors of s.  Locals of processes in `allow`, then for every
@@ -294,7 +296,7 @@ static int enumerate(const orc_model *m, const uint32_t *s, const sview *v, uint
   static __thread uint32_t tg[ORC_MAX_PROCS][MAXT]; /* per thread: orc_explore_mt */
   uint32_t tn[ORC_MAX_PROCS];
   memset(info, 0, sizeof *info);
-  for (uint32_t p = 0; p < N; p++) info->cluster0[p] = 1ull << p;
+  for (uint32_t p = 0; p < N; p++) info->cluster0[p] = info->stub[p] = 1ull << p;
   /* local successors */
   for (uint32_t p = 0; p < N; p++) {
     uint32_t c = entry_capacity(m, p);
@@ -306,6 +308,7 @@ static int enumerate(const orc_model *m, const uint32_t *s, const sview *v, uint
         uint32_t x = entry_target(m, ent, i, p);
         if (x == 0) break;
         info->local_succ |= 1ull << p;
+        info->nloc[p]++;
         if (!((allow >> p) & 1)) continue;
         memcpy(t, s, W * sizeof(uint32_t));
         orc_set(m, t, p, x - 1);
@@ -330,8 +333,15 @@ static int enumerate(const orc_model *m, const uint32_t *s, const sview *v, uint
       for (uint32_t f = 0; f < per; f++) {
         uint64_t r = getbits(m->syncbits[j], f * N, N);
         if (r == 0) continue;
+        /* a member that has the label: a rule all members have joins it
+         * whole; otherwise its lowest member without the label (the moves
+         * that can enable the rule) */
+        const uint64_t lack = r & ~en;
         for (uint32_t p = 0; p < N; p++)
-          if (((r >> p) & 1) && ((en >> p) & 1)) info->cluster0[p] |= r;
+          if (((r >> p) & 1) && ((en >> p) & 1)) {
+            info->cluster0[p] |= r;
+            info->stub[p] |= lack ? (lack & (~lack + 1)) : r;
+          }
         if (r & ~en) continue; /* not enabled */
         info->any_enabled_rule = 1;
         /* product of target lists, lowest process varies fastest (cu:1084-1113) */
@@ -431,6 +441,67 @@ static uint64_t select_cluster(uint32_t N, const uint64_t *cluster0, uint64_t pr
     }
   }
   return C;
+}
+
+/* Ample-set method: the local moves of one process p whose state has no
+ * synchronising entry with a rule containing p (cluster0(p) = {p}): no
+ * other process can enable a transition dependent on them.  Candidates must
+ * be in `cand` (a local successor; a new one under the cycle proviso); the
+ * candidate with the fewest local successors is taken, the lowest on ties;
+ * none gives full expansion. */
+static uint64_t select_ample(uint32_t N, const sinfo *info, uint64_t cand) {
+  uint64_t all = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
+  uint32_t best = 0xFFFFFFFFu;
+  uint64_t C = all;
+  for (uint32_t p = 0; p < N; p++) {
+    if (!((cand >> p) & 1) || info->cluster0[p] != (1ull << p)) continue;
+    if (info->nloc[p] < best) {
+      best = info->nloc[p];
+      C = 1ull << p;
+    }
+  }
+  return C;
+}
+
+/* Stubborn-set method over processes.  d(p) = {p} plus, for every rule r
+ * containing p of a label p has: r itself when every member has the label,
+ * else r's lowest member without it (a necessary enabling set).  K(p) is
+ * the closure of {p} under d; the transitions of K(p) (local moves of its
+ * processes, rules meeting it) form a stubborn set: an enabled rule meeting
+ * K lies inside K, a disabled one has a blocking process in K.  K(p) is
+ * valid when it meets `target` (processes with an enabled transition; with
+ * a new successor under the cycle proviso); the smallest valid K is taken,
+ * the lowest p on ties; none, or one of all N processes, gives full
+ * expansion. */
+static uint64_t select_stubborn(uint32_t N, const sinfo *info, uint64_t target) {
+  uint64_t all = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
+  int best = (int)N;
+  uint64_t C = all;
+  for (uint32_t p = 0; p < N; p++) {
+    uint64_t K = 1ull << p, done = 0;
+    while (K & ~done) {
+      uint32_t i = (uint32_t)__builtin_ctzll(K & ~done);
+      done |= 1ull << i;
+      K |= info->stub[i];
+    }
+    if (!(K & target)) continue;
+    int pc = __builtin_popcountll(K);
+    if (pc < best) {
+      best = pc;
+      C = K;
+    }
+  }
+  return C;
+}
+
+uint64_t orc_por_select(const orc_model *m, const uint32_t *state, int method) {
+  sview v;
+  sinfo info;
+  if (view_state(m, state, &v)) return 0;
+  if (enumerate(m, state, &v, 0, NULL, NULL, &info)) return 0;
+  if (method == ORC_POR_AMPLE) return select_ample(m->nr_procs, &info, info.local_succ);
+  if (method == ORC_POR_STUBBORN) return select_stubborn(m->nr_procs, &info, info.local_succ | info.in_enabled);
+  return select_cluster(m->nr_procs, info.cluster0, info.local_succ | info.in_enabled);
 }
 
 uint64_t orc_por_cluster(const orc_model *m, const uint32_t *state) {
@@ -647,7 +718,7 @@ int orc_explore(const orc_model *m, const orc_options *o, orc_result *res) {
         /* full expansion pass: enabledness, clusters, safety */
         rc = enumerate(m, s, &v, ~0ull, on_full, &b, &info);
         if (rc) goto out;
-        uint64_t prov;
+        uint64_t prov, lnew = 0;
         if (!o->cycle_proviso) {
           prov = info.local_succ | info.in_enabled; /* cu:1395-1398, 1527-1530 */
         } else {
@@ -656,10 +727,16 @@ int orc_explore(const orc_model *m, const orc_options *o, orc_result *res) {
           b.nrules_new = 0;
           rc = enumerate(m, s, &v, ~0ull, on_proviso, &b, &tmp);
           if (rc) goto out;
-          prov = b.local_new;
+          prov = lnew = b.local_new;
           for (uint64_t r = 0; r < b.nrules_new; r++) prov |= b.rules_new[r];
         }
-        uint64_t C = select_cluster(m->nr_procs, info.cluster0, prov);
+        uint64_t C;
+        if (o->por_method == ORC_POR_AMPLE)
+          C = select_ample(m->nr_procs, &info, o->cycle_proviso ? lnew : info.local_succ);
+        else if (o->por_method == ORC_POR_STUBBORN)
+          C = select_stubborn(m->nr_procs, &info, prov);
+        else
+          C = select_cluster(m->nr_procs, info.cluster0, prov);
         sinfo info2;
         rc = enumerate(m, s, &v, C, on_succ, &b, &info2);
         if (rc) goto out;

@@ -78,12 +78,17 @@ enum {
   ORC_ERR_ARG = -6
 };
 
+/* partial-order reduction methods (the reference keeps each on its own
+ * branch, README.md lines 24-26; this snapshot holds cample only) */
+enum { ORC_POR_CAMPLE = 0, ORC_POR_AMPLE = 1, ORC_POR_STUBBORN = 2 };
+
 typedef struct orc_options {
   int property;       /* ORC_PROP_* */
   int por;
   int cycle_proviso;
   uint64_t max_states; /* 0 = unbounded; else stop after the level that crosses it */
   uint64_t hash_seed;
+  int por_method;     /* ORC_POR_*, used when por is set */
 } orc_options;
 
 typedef struct orc_result {
@@ -112,6 +117,10 @@ long orc_successors(const orc_model *m, const uint32_t *state, uint32_t *out,
 /* cample cluster selected for `state` without the cycle proviso (bitmask of
  * processes; all-ones if no reduction applies). */
 uint64_t orc_por_cluster(const orc_model *m, const uint32_t *state);
+
+/* process set selected for `state` by ORC_POR_* `method` without the cycle
+ * proviso (all-ones if no reduction applies) */
+uint64_t orc_por_select(const orc_model *m, const uint32_t *state, int method);
 
 int orc_explore(const orc_model *m, const orc_options *opt, orc_result *res);
 

@@ -15,7 +15,7 @@
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s <model> [-d|-p] [--por] [--cycle-proviso] [--max-states N]\n", argv[0]);
+    fprintf(stderr, "usage: %s <model> [-d|-p] [--por] [--por-method cample|ample|stubborn] [--cycle-proviso] [--max-states N]\n", argv[0]);
     return 1;
   }
   orc_options o;
@@ -25,6 +25,17 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[i], "-p")) o.property = ORC_PROP_SAFETY;
     else if (!strcmp(argv[i], "--por")) o.por = 1;
     else if (!strcmp(argv[i], "--cycle-proviso")) o.cycle_proviso = 1;
+    else if (!strcmp(argv[i], "--por-method") && i + 1 < argc) {
+      const char *v = argv[++i];
+      o.por = 1;
+      if (!strcmp(v, "cample")) o.por_method = ORC_POR_CAMPLE;
+      else if (!strcmp(v, "ample")) o.por_method = ORC_POR_AMPLE;
+      else if (!strcmp(v, "stubborn")) o.por_method = ORC_POR_STUBBORN;
+      else {
+        fprintf(stderr, "unknown POR method %s\n", v);
+        return 1;
+      }
+    }
     else if (!strcmp(argv[i], "--max-states") && i + 1 < argc) o.max_states = strtoull(argv[++i], NULL, 10);
     else if (!strcmp(argv[i], "--seed") && i + 1 < argc) o.hash_seed = strtoull(argv[++i], NULL, 10);
     else {

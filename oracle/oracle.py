@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
 PROP_NONE, PROP_DEADLOCK, PROP_SAFETY, PROP_LIVENESS = 0, 1, 2, 3
+POR_CAMPLE, POR_AMPLE, POR_STUBBORN = 0, 1, 2
 
 
 class _Opts(ctypes.Structure):
@@ -25,6 +26,7 @@ class _Opts(ctypes.Structure):
         ("cycle_proviso", ctypes.c_int),
         ("max_states", ctypes.c_uint64),
         ("hash_seed", ctypes.c_uint64),
+        ("por_method", ctypes.c_int),
     ]
 
 
@@ -96,6 +98,8 @@ def lib():
                               ctypes.c_uint32]
         L.orc_por_cluster.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(ctypes.c_uint32)]
         L.orc_por_cluster.restype = ctypes.c_uint64
+        L.orc_por_select.argtypes = [ctypes.POINTER(_Model), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+        L.orc_por_select.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -169,9 +173,12 @@ class OracleModel:
     def por_cluster(self, words: List[int]) -> int:
         return lib().orc_por_cluster(self.ptr, self._vec(words))
 
+    def por_select(self, words: List[int], method: int) -> int:
+        return lib().orc_por_select(self.ptr, self._vec(words), method)
+
     def explore(self, prop: int = PROP_NONE, por: bool = False, proviso: bool = False,
-                max_states: int = 0, seed: int = 0) -> dict:
-        o = _Opts(prop, int(por), int(proviso), max_states, seed)
+                max_states: int = 0, seed: int = 0, method: int = POR_CAMPLE) -> dict:
+        o = _Opts(prop, int(por), int(proviso), max_states, seed, method)
         r = _Res()
         rc = lib().orc_explore(self.ptr, ctypes.byref(o), ctypes.byref(r))
         if rc != 0:
@@ -182,7 +189,7 @@ class OracleModel:
 
     def explore_mt(self, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:
         """orc_explore_mt: the plain method on `threads` threads."""
-        o = _Opts(prop, 0, 0, max_states, seed)
+        o = _Opts(prop, 0, 0, max_states, seed, POR_CAMPLE)
         r = _Res()
         rc = lib().orc_explore_mt(self.ptr, ctypes.byref(o), threads, ctypes.byref(r))
         if rc != 0:
@@ -193,8 +200,8 @@ class OracleModel:
 
 
 def explore(path_noext: str, prop: int = PROP_NONE, por: bool = False, proviso: bool = False,
-            max_states: int = 0, seed: int = 0) -> dict:
-    return OracleModel(path_noext, por).explore(prop, por, proviso, max_states, seed)
+            max_states: int = 0, seed: int = 0, method: int = 0) -> dict:
+    return OracleModel(path_noext, por).explore(prop, por, proviso, max_states, seed, method)
 
 
 def explore_mt(path_noext: str, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:

@@ -162,7 +162,48 @@ class NaiveModel:
                 best, C = len(fin[p]), fin[p]
         return C
 
-    def explore(self, prop: str = "", por: bool = False, proviso: bool = False):
+    def stub_relation(self, s: Tuple[int, ...]):
+        """d(p) of the stubborn-set method: p, plus for each rule r containing
+        p of a label p has at s_p, r itself when every member has the label,
+        else r's lowest member without it."""
+        n = self.n
+        ent = [self.entries(p, s[p]) for p in range(n)]
+        d = [{p} for p in range(n)]
+        for p in range(n):
+            for a in ent[p][1]:
+                for r in self.rules[a]:
+                    if not (r >> p) & 1:
+                        continue
+                    members = [q for q in range(n) if (r >> q) & 1]
+                    lacking = [q for q in members if a not in ent[q][1]]
+                    d[p] |= set(members) if not lacking else {min(lacking)}
+        return d
+
+    @staticmethod
+    def select_ample(n, cluster0, nloc, cand):
+        best, C = None, set(range(n))
+        for p in range(n):
+            if p in cand and cluster0[p] == {p} and (best is None or nloc[p] < best):
+                best, C = nloc[p], {p}
+        return C
+
+    @staticmethod
+    def select_stubborn(n, d, target):
+        best, C = n, set(range(n))
+        for p in range(n):
+            K = {p}
+            while True:
+                K2 = set(K)
+                for i in K:
+                    K2 |= d[i]
+                if K2 == K:
+                    break
+                K = K2
+            if K & target and len(K) < best:
+                best, C = len(K), K
+        return C
+
+    def explore(self, prop: str = "", por: bool = False, proviso: bool = False, method: str = "cample"):
         n = self.n
         s0 = tuple([0] * n)
         level = {s0: 0}
@@ -179,16 +220,25 @@ class NaiveModel:
                 if prop == "p" and any(t[n - 1] == 1 for t, _ in succ):
                     violation = True
                 if por:
+                    lnew = set()
                     if proviso:
                         prov = set()
                         for t, (kind, who) in succ:
                             if not (t in level and level[t] <= lv):
                                 prov |= {who} if kind == "L" else set(who)
+                                if kind == "L":
+                                    lnew.add(who)
                     else:
                         prov = set(has_local)
                         for _, members in enabled:
                             prov |= members
-                    C = self.select(n, cl0, prov)
+                    if method == "ample":
+                        nloc = [sum(1 for _, (k, w) in succ if k == "L" and w == p) for p in range(n)]
+                        C = self.select_ample(n, cl0, nloc, lnew if proviso else set(has_local))
+                    elif method == "stubborn":
+                        C = self.select_stubborn(n, self.stub_relation(s), prov)
+                    else:
+                        C = self.select(n, cl0, prov)
                     amp = [t for t, (kind, who) in succ
                            if (kind == "L" and who in C) or (kind == "R" and who <= C)]
                 else:
