@@ -55,6 +55,7 @@ struct ModelHdr {
   uint32_t off_synx;       // per syn_tgt entry: the same for synchronising targets
   uint32_t off_fclr;       // per process: its field bits (key-shaped)
   uint32_t off_rclr;       // per rule: the field bits of its members (key-shaped)
+  uint32_t synlab_lo, synlab_hi;  // labels < 64 with a synchronising entry in some row
   uint32_t tps_begin;      // first word of the section the state-per-lane kernels read
                            // (rowbase .. rclr, 4-word aligned, up to total_words)
 };

@@ -131,6 +131,7 @@ int compile_model(const Gpf &g, Compiled *out) {
   rowflags.reserve(nrows);
   uint32_t max_label = 0;
   bool any_sync = false;
+  uint64_t synlab_all = 0;
   const uint32_t nt = static_cast<uint32_t>(g.proc_trans.size());
   for (uint32_t p = 0; p < N; p++) {
     const uint32_t bs = g.bits_state[p];
@@ -186,6 +187,7 @@ int compile_model(const Gpf &g, Compiled *out) {
         any_sync = true;
       }
       rowflags.push_back(flags);
+      synlab_all |= lm;
       rowlab.push_back(static_cast<uint32_t>(lm));
       rowlab.push_back(static_cast<uint32_t>(lm >> 32));
     }
@@ -196,6 +198,8 @@ int compile_model(const Gpf &g, Compiled *out) {
   // rules per label (cu:997-1009, 1466-1488)
   const uint32_t nlab = any_sync ? max_label + 1 : 0;
   h.nlab = nlab;
+  h.synlab_lo = static_cast<uint32_t>(synlab_all);
+  h.synlab_hi = static_cast<uint32_t>(synlab_all >> 32);
   h.lab64 = nlab <= 64 ? 1u : 0u;
   std::vector<uint32_t> rule_start, rules;
   const uint32_t k2 = 32 / g.nbits_syncbits_offset, per = 32 / N;

@@ -120,6 +120,21 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   TpsInfo r{};
   bool anyloc = false, anyen = false;
   uint32_t multi = 0;  // processes whose row has a multi-target synchronising entry
+  // A model with at most kTpsLabCache synchronising labels gets every
+  // label's enabled set in this pass (men[k] for its k-th label), so the
+  // processes' label masks are read once.
+  const uint64_t msyn = static_cast<uint64_t>(H.synlab_lo) | (static_cast<uint64_t>(H.synlab_hi) << 32);
+  const bool men_all = __popcll(msyn) <= kTpsLabCache;
+  uint32_t mlab[kTpsLabCache], men[kTpsLabCache];
+  {
+    uint64_t L = msyn;
+#pragma unroll
+    for (int k = 0; k < kTpsLabCache; k++) {
+      mlab[k] = L ? static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1) : 64u;
+      L &= L - 1;
+      men[k] = 0;
+    }
+  }
   for (int p = 0; p < N; p++) {
     uint32_t row = kNoRow;
     uint4 ra = make_uint4(0, 0, 0, 0);
@@ -137,7 +152,13 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
       r.cnt += ra.y - ra.x;
       anyloc |= (ra.w & kRowHasLocal) != 0;
       if (ra.w & kRowMultiSync) multi |= 1u << p;
-      r.labs |= tps_rowlab(M, H, row);
+      const uint64_t rl = tps_rowlab(M, H, row);
+      r.labs |= rl;
+      if (men_all) {
+#pragma unroll
+        for (int k = 0; k < kTpsLabCache; k++)
+          if (mlab[k] < 64u && ((rl >> mlab[k]) & 1ull)) men[k] |= 1u << p;
+      }
       if (safety && p == N - 1)
         for (uint32_t t = ra.x; t < ra.y; t++)
           if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) r.viol = true;
@@ -155,7 +176,14 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     }
     r.cached = (L == 0);
   }
-  if (r.cached && r.labs) {
+  if (r.cached && r.labs && men_all) {
+#pragma unroll
+    for (int i = 0; i < kTpsLabCache; i++) {
+      if (lab[i] >= 64u) continue;
+      const int k = __popcll(msyn & ((1ull << lab[i]) - 1ull));  // lab[i] is the model's k-th label
+      r.en[i] = k == 0 ? men[0] : k == 1 ? men[1] : k == 2 ? men[2] : men[3];
+    }
+  } else if (r.cached && r.labs) {
     for (int p = 0; p < N; p++) {
       const uint32_t row = row_at(rows, p * 64 + lane);
       if (row == kNoRow) continue;
