@@ -79,6 +79,34 @@ def batched_vs_plain(gx, gen, tmp: str) -> dict:
     }
 
 
+def por_reduction(gx, gen, tmp: str) -> dict:
+    """config 3 (24 processes, 8 states, phased rounds): the full state space
+    once, then every POR method with no property, with -d and with the cycle
+    proviso; each reduced run is the best of three."""
+    p = os.path.join(tmp, "config3")
+    gen.config3(3).write(p)
+    full_model = gx.Model.load(p)
+    ex = gx.Explorer(full_model, q_ints=1 << 34, property=gx.PROP_NONE)
+    full = ex.run()
+    del ex
+    por_model = gx.Model.load(p, por=True)
+    out = {"workload": "config3: 24 LTSs x 8 states, 72-bit vectors, phased rounds (70/20/10 local/rendezvous/barrier)",
+           "full": {"states": full["states"], "transitions": full["transitions"], "t_s": full["t_explore_s"]},
+           "methods": {}}
+    for name, method in (("cample", gx.POR_CAMPLE), ("ample", gx.POR_AMPLE), ("stubborn", gx.POR_STUBBORN)):
+        row = {}
+        for leg, prop, prov in (("none", gx.PROP_NONE, 0), ("deadlock", gx.PROP_DEADLOCK, 0), ("proviso", gx.PROP_NONE, 1)):
+            ex = gx.Explorer(por_model, q_ints=1 << 26, property=prop, por=1, cycle_proviso=prov, por_method=method)
+            rs = [ex.run() for _ in range(3)]
+            r = min(rs, key=lambda x: x["t_explore_s"])
+            row[leg] = {"states": r["states"], "transitions": r["transitions"], "levels": r["levels"],
+                        "violation": bool(r["violation"]), "t_s": r["t_explore_s"]}
+            del ex
+        row["reduction"] = full["states"] / row["none"]["states"]
+        out["methods"][name] = row
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +120,7 @@ def main() -> int:
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--threads", type=int, default=0, help="successor-kernel workgroup size, 0 = library default")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
+    ap.add_argument("--no-por", action="store_true", help="skip the config 3 partial-order reduction runs")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson job stream through the server (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="jobs per second of the stream")
@@ -156,6 +185,10 @@ def main() -> int:
                 traffic = v["bytes_per_launch"]
 
     batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp)
+    por = None
+    if not args.no_por:
+        del ex  # the config 2 table (2^34 ints) is not needed any more
+        por = por_reduction(gx, gen, tmp)
 
     serving = None
     if args.serve_seconds > 0:
@@ -235,6 +268,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched_vs_plain": batch,
+        "por": por,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -291,34 +291,34 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
         }
         wave_sync();
       } else {
-      const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
-      xc[lane] = c;
-      wave_sync();
-      uint32_t K = c, done = 0;
-      bool ok = false;
-      if (in_group) {
-        while (K & ~done) {  // closure (cu:1641-1649)
-          const int i = __ffs(K & ~done) - 1;
-          done |= 1u << i;
-          const uint32_t ci = xc[base + i];
-          if ((ci >> i) & 1u) ok = true;
-          K |= ci;
-        }
-      }
-      xf[lane] = ok ? K : 0u;  // cu:1651-1655
-      wave_sync();
-      if (in_group) {
-        int best = N;
-        for (int q = 0; q < N; q++) {  // first smallest non-empty cluster (cu:1657-1666)
-          const uint32_t fq = xf[base + q];
-          const int pc = __popc(fq);
-          if (fq && pc < best) {
-            best = pc;
-            csel = fq;
+        const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
+        xc[lane] = c;
+        wave_sync();
+        uint32_t K = c, done = 0;
+        bool ok = false;
+        if (in_group) {
+          while (K & ~done) {  // closure (cu:1641-1649)
+            const int i = __ffs(K & ~done) - 1;
+            done |= 1u << i;
+            const uint32_t ci = xc[base + i];
+            if ((ci >> i) & 1u) ok = true;
+            K |= ci;
           }
         }
-      }
-      wave_sync();
+        xf[lane] = ok ? K : 0u;  // cu:1651-1655
+        wave_sync();
+        if (in_group) {
+          int best = N;
+          for (int q = 0; q < N; q++) {  // first smallest non-empty cluster (cu:1657-1666)
+            const uint32_t fq = xf[base + q];
+            const int pc = __popc(fq);
+            if (fq && pc < best) {
+              best = pc;
+              csel = fq;
+            }
+          }
+        }
+        wave_sync();
       }
     }
 
