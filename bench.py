@@ -263,6 +263,9 @@ def main() -> int:
         "hash_phase": {
             "lookups_per_s": lookups / t_hash,
             "inserts_per_s": inserts / t_hash,
+            # each insert claims its slot with one 64-bit atomicCAS; attempts
+            # lost to a racing insert into the same slot are not counted
+            "cas_per_s": inserts / t_hash,
             "t_hash_s_per_step": t_hash / args.steps,
             "t_expand_s_per_step": sum(r["t_expand_s"] for r in runs) / args.steps,
         },
