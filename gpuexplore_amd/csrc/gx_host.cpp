@@ -173,10 +173,12 @@ struct gx_explorer {
   uint64_t nbuckets = 0, table_bytes = 0, front_cap = 0, cand_cap = 0;
   DevBuf d_table, d_front0, d_front1, d_cand, d_retry0, d_retry1, d_ctrl, d_model, d_count;
   hipStream_t st = nullptr;
-  Events evs;
+  Events evs[2];        // per batch of levels in flight
+  Ctrl *h_ctrl = nullptr;  // pinned: the control block after each batch, two slots
   Ctrl last{};       // control block at the end of the last run
   bool ran = false;
   ~gx_explorer() {
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -305,6 +307,7 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   GX_TRY(e->d_ctrl.alloc(sizeof(Ctrl)));
   GX_TRY(e->d_model.alloc(m->comp.words.size() * 4));
   GX_TRY(e->d_count.alloc(8));
+  GX_TRY(hipHostMalloc(reinterpret_cast<void **>(&e->h_ctrl), 2 * sizeof(Ctrl), hipHostMallocDefault));
   GX_TRY(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
   GX_TRY(hipMemcpyAsync(e->d_model.p, m->comp.words.data(), m->comp.words.size() * 4, hipMemcpyHostToDevice,
                         e->st));
@@ -426,29 +429,38 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   res->t_init_s = std::chrono::duration<double>(t_init1 - t_init0).count();
 
   const uint32_t kiters = std::max<uint32_t>(1, opt->kernel_iters);
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ins_ev, exp_ev;
+  // Without per-level output the host keeps one more batch of levels
+  // enqueued while it reads the control block of the previous one, so the
+  // GPU does not idle through the read-back.  A batch enqueued after the
+  // last level finds `stop` set: every level kernel then returns at once.
+  const bool ahead = opt->verbosity == 0;
+  struct Batch {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ins, exp;
+    hipEvent_t done = nullptr;
+  } batch[2];
   double hash_ms = 0, exp_ms = 0;
   Ctrl ch{};
-  double front = 1.0, degree = 16.0;  // known at each host sync
-  const auto t0 = std::chrono::steady_clock::now();
-  for (;;) {
-    e->evs.used = 0;
+  // frontier and out-degree estimates for the launch geometry
+  double front = 1.0, prev_front = 1.0, degree = 16.0;
+  auto enqueue = [&](int k, double front_est) -> int {
+    Batch &b = batch[k];
+    e->evs[k].used = 0;
+    b.ins.clear();
+    b.exp.clear();
     const uint32_t ins_blocks = opt->insert_blocks ? opt->insert_blocks
-                                : kiters == 1 ? insert_grid(e->cus, front, degree)
+                                : kiters == 1 ? insert_grid(e->cus, front_est, degree)
                                               : e->cus * kInsertBlocksPerCU;
     const uint32_t exp_blocks = (opt->blocks || kiters != 1) ? e->blocks
-                                : expand_grid(e->cus, e->threads, e->blocks, front);
-    ins_ev.clear();
-    exp_ev.clear();
+                                : expand_grid(e->cus, e->threads, e->blocks, front_est);
     for (uint32_t it = 0; it < kiters; it++) {
-      hipEvent_t a0 = e->evs.next(), a1 = e->evs.next(), b0 = e->evs.next(), b1 = e->evs.next();
+      hipEvent_t a0 = e->evs[k].next(), a1 = e->evs[k].next(), b0 = e->evs[k].next(), b1 = e->evs[k].next();
       if (!a0 || !a1 || !b0 || !b1) return GX_E_HIP;
       if (e->fused) {  // one launch per level: its time is hash time
         GX_TRY(hipEventRecord(b0, st));
         GX_TRY(launch_fused(fa, W, BW, e->lds, exp_blocks, 256, st));
         GX_TRY(hipEventRecord(b1, st));
         GX_TRY(launch_advance(ia.ctrl, st));
-        ins_ev.push_back({b0, b1});
+        b.ins.push_back({b0, b1});
         continue;
       }
       GX_TRY(hipEventRecord(a0, st));
@@ -458,22 +470,51 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
       GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, ins_blocks, kInsertThreads, st));
       GX_TRY(hipEventRecord(b1, st));
       GX_TRY(launch_advance(ia.ctrl, st));
-      exp_ev.push_back({a0, a1});
-      ins_ev.push_back({b0, b1});
+      b.exp.push_back({a0, a1});
+      b.ins.push_back({b0, b1});
     }
-    GX_TRY(hipMemcpyAsync(&ch, e->d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
-    GX_TRY(hipStreamSynchronize(st));
-    for (auto &p : exp_ev) {
+    GX_TRY(hipMemcpyAsync(e->h_ctrl + k, e->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    b.done = e->evs[k].next();
+    if (!b.done) return GX_E_HIP;
+    GX_TRY(hipEventRecord(b.done, st));
+    return GX_OK;
+  };
+  auto collect = [&](int k) -> int {
+    Batch &b = batch[k];
+    GX_TRY(hipEventSynchronize(b.done));
+    for (auto &p : b.exp) {
       float ms = 0;
       GX_TRY(hipEventElapsedTime(&ms, p.first, p.second));
       exp_ms += ms;
     }
-    for (auto &p : ins_ev) {
+    for (auto &p : b.ins) {
       float ms = 0;
       GX_TRY(hipEventElapsedTime(&ms, p.first, p.second));
       hash_ms += ms;
     }
-    res->hash_launches += ins_ev.size();
+    res->hash_launches += b.ins.size();
+    return GX_OK;
+  };
+  // the frontier one batch further on, extrapolated from the last growth
+  auto ahead_front = [&]() {
+    const double g = std::min(8.0, std::max(0.125, front / std::max(1.0, prev_front)));
+    return std::max(1.0, front * std::pow(g, static_cast<double>(kiters)));
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  int cur = 0;
+  {
+    const int rc = enqueue(0, front);
+    if (rc != GX_OK) return rc;
+  }
+  if (ahead) {
+    const int rc = enqueue(1, front);
+    if (rc != GX_OK) return rc;
+  }
+  for (;;) {
+    int rc = collect(cur);
+    if (rc != GX_OK) return rc;
+    ch = e->h_ctrl[cur];
+    prev_front = front;
     front = static_cast<double>(ch.front_count[ch.cur]);
     if (ch.expanded) degree = static_cast<double>(ch.transitions) / static_cast<double>(ch.expanded);
     if (opt->verbosity == 1) printf("%u\n", ch.levels_done);  // cu:2129-2131
@@ -490,7 +531,21 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
       const int prc = print_states(e, stdout, ch);
       if (prc != GX_OK) return prc;
     }
-    if (ch.stop) break;
+    if (ch.stop) {
+      if (ahead) {  // the batch behind it found stop set and did nothing
+        rc = collect(cur ^ 1);
+        if (rc != GX_OK) return rc;
+      }
+      break;
+    }
+    if (ahead) {
+      rc = enqueue(cur, ahead_front());
+      if (rc != GX_OK) return rc;
+      cur ^= 1;
+    } else {
+      rc = enqueue(cur, front);
+      if (rc != GX_OK) return rc;
+    }
   }
   const auto t1 = std::chrono::steady_clock::now();
   res->t_explore_s = std::chrono::duration<double>(t1 - t0).count();
