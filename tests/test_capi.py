@@ -100,3 +100,26 @@ def test_server_options_default(gx):
     gx.lib().gx_server_options_default(o)
     assert o.slots == 256 and o.key_words == 2 and o.rounds_per_step >= 1
     assert o.table_bytes % o.page_bytes == 0
+
+
+@pytest.mark.parametrize("cls,fn", [("Options", "gx_options_default"),
+                                    ("BatchOptions", "gx_batch_options_default"),
+                                    ("ServerOptions", "gx_server_options_default")])
+def test_option_struct_layouts(gx, cls, fn):
+    """the ctypes mirrors have the C layouts: the C defaults function writes
+    exactly the mirror's bytes (guard bytes after it untouched) and every
+    field, the last one included (the struct was filled with 0xFF first)"""
+    S = getattr(gx, cls)
+
+    class Guarded(ctypes.Structure):
+        _fields_ = [("o", S), ("guard", ctypes.c_uint8 * 64)]
+
+    g = Guarded()
+    ctypes.memset(ctypes.addressof(g), 0xFF, ctypes.sizeof(S))
+    ctypes.memset(ctypes.addressof(g.guard), 0xA5, 64)
+    getattr(gx.lib(), fn)(ctypes.byref(g.o))
+    assert list(g.guard) == [0xA5] * 64
+    last = S._fields_[-1][0]
+    assert getattr(g.o, last) != (1 << (8 * ctypes.sizeof(S._fields_[-1][1]))) - 1, last
+    if cls == "Options":
+        assert g.o.kernel_iters == 1 and g.o.por_method == gx.POR_CAMPLE and g.o.hash_seed != 0

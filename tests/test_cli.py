@@ -51,3 +51,12 @@ def test_no_device(gx):
     r = run(gx, os.path.join(GOLDEN, "tiny2"), "-d")
     assert r.returncode == 1
     assert "No HIP compatible GPU detected" in r.stderr
+
+
+def test_unknown_por_method(gx, model_dir):
+    from gpuexplore_amd import gen
+    p = os.path.join(model_dir, "m")
+    gen.config1(1).write(p)
+    r = subprocess.run([gx.CLI_PATH, p, "--por-method", "sleep-sets"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "unknown partial-order reduction method sleep-sets" in r.stderr
