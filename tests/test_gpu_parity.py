@@ -238,3 +238,23 @@ def test_edge_models(gpu, oracle_mod, model_dir, name):
                     assert ek == 3 and e.status == -7  # GX_E_ARG: no state-per-lane kernel for > 64 labels
                     continue
                 assert same(a, b), (name, prop, por, prov, ek, a, b)
+
+
+@pytest.mark.parametrize("kiters", [1, 2, 3, 5])
+def test_stop_with_levels_enqueued_ahead(gpu, oracle_mod, model_dir, kiters):
+    """the host keeps levels enqueued beyond the one it reads; a run that
+    stops (deadlock found, state bound reached, frontier empty) still
+    reports exactly the oracle's level and counts"""
+    p = os.path.join(model_dir, "chain")
+    gen.independent_chains(3, 3).write(p)
+    for prop in (0, 1):
+        a = oracle_mod.explore(p, prop)
+        say("ahead chain", kiters, prop)
+        b = gpu_run(gpu, p, prop, kernel_iters=kiters)
+        assert same(a, b), (kiters, prop, a, b)
+    p = os.path.join(model_dir, "c1")
+    gen.config1(3).write(p)
+    a = oracle_mod.explore(p, 0, max_states=2000)
+    say("ahead truncated", kiters)
+    b = gpu_run(gpu, p, 0, kernel_iters=kiters, max_states=2000)
+    assert same(a, b) and a["truncated"] and b["truncated"], (kiters, a, b)
