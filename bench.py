@@ -119,6 +119,7 @@ def main() -> int:
     ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane, 3 fused level")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--threads", type=int, default=0, help="successor-kernel workgroup size, 0 = library default")
+    ap.add_argument("--insert-blocks", type=int, default=0, help="FIND-OR-PUT workgroups per launch, 0 = sized per level")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-por", action="store_true", help="skip the config 3 partial-order reduction runs")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -147,7 +148,8 @@ def main() -> int:
     gx.lib().gx_bench_copy(4 << 30, 20, ctypes.byref(peak))
 
     ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE,
-                     expand_kernel=args.expand_kernel, bucket_bytes=args.bucket_bytes, threads=args.threads)
+                     expand_kernel=args.expand_kernel, bucket_bytes=args.bucket_bytes, threads=args.threads,
+                     insert_blocks=args.insert_blocks)
     for i in range(args.warmup):
         r = ex.run()
         print(f"warmup {i}: states={r['states']} t_explore={r['t_explore_s']:.3f}s", file=sys.stderr, flush=True)
@@ -245,6 +247,7 @@ def main() -> int:
             "key_words": info["key_words"],
             "bucket_bytes": args.bucket_bytes,
             "threads": args.threads,
+            "insert_blocks": args.insert_blocks,
             "expand_kernel": ["auto", "process-groups", "state-per-lane", "fused"][args.expand_kernel],
         },
         "roofline": {
