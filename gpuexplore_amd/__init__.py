@@ -249,11 +249,20 @@ class Model:
             self.h = None
 
 
+def _set_fields(o: ctypes.Structure, kw: dict) -> None:
+    """Sets option fields by name; an unknown name is an error, not a new
+    Python attribute that the library would never see."""
+    names = {f for f, _ in type(o)._fields_}
+    for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"{type(o).__name__} has no field {k!r}")
+        setattr(o, k, v)
+
+
 def options(**kw) -> Options:
     o = Options()
     lib().gx_options_default(ctypes.byref(o))
-    for k, v in kw.items():
-        setattr(o, k, v)
+    _set_fields(o, kw)
     return o
 
 
@@ -304,8 +313,7 @@ def explore_batch(models, properties=None, weights=None, **kw):
     n = len(models)
     o = BatchOptions()
     lib().gx_batch_options_default(ctypes.byref(o))
-    for k, v in kw.items():
-        setattr(o, k, v)
+    _set_fields(o, kw)
     hs = (ctypes.c_void_p * n)(*[m.h.value for m in models])
     props = (ctypes.c_int32 * n)(*(properties or [0] * n))
     w = (ctypes.c_uint64 * n)(*weights) if weights is not None else None
@@ -325,8 +333,7 @@ class Server:
     def __init__(self, **kw):
         o = ServerOptions()
         lib().gx_server_options_default(ctypes.byref(o))
-        for k, v in kw.items():
-            setattr(o, k, v)
+        _set_fields(o, kw)
         h = ctypes.c_void_p()
         _check(lib().gx_server_create(ctypes.byref(o), ctypes.byref(h)), "server")
         self.h = h

@@ -123,3 +123,14 @@ def test_option_struct_layouts(gx, cls, fn):
     assert getattr(g.o, last) != (1 << (8 * ctypes.sizeof(S._fields_[-1][1]))) - 1, last
     if cls == "Options":
         assert g.o.kernel_iters == 1 and g.o.por_method == gx.POR_CAMPLE and g.o.hash_seed != 0
+
+
+def test_unknown_option_names_are_errors(gx):
+    """a misspelled option must not be silently ignored"""
+    assert gx.options(por_method=gx.POR_STUBBORN, q_ints=1 << 20).por_method == gx.POR_STUBBORN
+    with pytest.raises(TypeError):
+        gx.options(por_methd=2)
+    with pytest.raises(TypeError):
+        gx.explore_batch([], [], not_a_field=1)
+    with pytest.raises(TypeError):
+        gx.Server(slot=4)
