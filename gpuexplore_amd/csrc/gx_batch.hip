@@ -424,6 +424,7 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
     inst[i].hdr = H;
     inst[i].model_off = static_cast<uint32_t>(mbuf.size());
     inst[i].property = properties ? properties[i] : GX_PROP_NONE;
+    if (inst[i].property < GX_PROP_NONE || inst[i].property > GX_PROP_LIVENESS) return GX_E_ARG;
     inst[i].cur = 1;
     inst[i].level = 0xFFFFFFFFu;
     mbuf.insert(mbuf.end(), m->comp.words.begin(), m->comp.words.end());

@@ -249,6 +249,7 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   const int W = static_cast<int>(H.key_words);
   const bool proviso = opt->por && opt->cycle_proviso;
   if (opt->por_method > GX_POR_STUBBORN) return GX_E_ARG;
+  if (opt->property < GX_PROP_NONE || opt->property > GX_PROP_LIVENESS) return GX_E_ARG;
   const uint32_t K = H.state_bits;
   const int WT = proviso ? std::max<int>(1, static_cast<int>((K + 32 + 62) / 63)) : W;
   if (WT > kMaxKeyWords || W > kMaxKeyWords) return GX_E_LIMIT;

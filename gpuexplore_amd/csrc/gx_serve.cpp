@@ -194,6 +194,7 @@ extern "C" void gx_server_free(gx_server *s) { delete s; }
 extern "C" int gx_server_submit(gx_server *s, const gx_model *m, int32_t property, uint64_t est_states,
                                 uint64_t *job_id) {
   if (!s || !m || m->por) return GX_E_ARG;
+  if (property < GX_PROP_NONE || property > GX_PROP_LIVENESS) return GX_E_ARG;
   const ModelHdr &H = m->comp.hdr;
   if (!H.lab64 || H.key_words > static_cast<uint32_t>(s->W) || H.nprocs > s->o.max_procs ||
       m->comp.words.size() > s->model_words)

@@ -109,3 +109,20 @@ def test_bad_method_rejected(gpu, model_dir):
     with pytest.raises(gpu.GxError) as e:
         gpu_run(gpu, p, 0, True, False, por_method=3)
     assert e.value.status == -7  # GX_E_ARG
+
+
+def test_bad_property_rejected(gpu, model_dir):
+    """property codes outside GX_PROP_* are errors at every entry point"""
+    p = os.path.join(model_dir, "triad")
+    triad().write(p)
+    m = gpu.Model.load(p)
+    with pytest.raises(gpu.GxError) as e:
+        gpu.explore(m, property=7, q_ints=1 << 20)
+    assert e.value.status == -7
+    with pytest.raises(gpu.GxError) as e:
+        gpu.explore_batch([m], [-1], q_ints=1 << 20)
+    assert e.value.status == -7
+    srv = gpu.Server(slots=4, table_bytes=1 << 26, page_bytes=1 << 16)
+    with pytest.raises(gpu.GxError) as e:
+        srv.submit(m, 9, est_states=10)
+    assert e.value.status == -7
