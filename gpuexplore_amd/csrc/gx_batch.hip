@@ -393,6 +393,7 @@ int map_err(hipError_t e) {
 }  // namespace
 
 extern "C" void gx_batch_options_default(gx_batch_options *o) {
+  if (!o) return;
   memset(o, 0, sizeof *o);
   o->threads = 256;
   o->levels_per_sync = 8;

@@ -135,6 +135,7 @@ extern "C" int gx_model_info_get(const gx_model *m, gx_model_info *o) {
 }
 
 extern "C" void gx_options_default(gx_options *o) {
+  if (!o) return;
   memset(o, 0, sizeof *o);
   o->kernel_iters = 1;
   o->bucket_bytes = 0;

@@ -89,6 +89,7 @@ struct gx_server {
 };
 
 extern "C" void gx_server_options_default(gx_server_options *o) {
+  if (!o) return;
   memset(o, 0, sizeof *o);
   o->slots = 256;
   o->key_words = 2;

@@ -134,3 +134,17 @@ def test_unknown_option_names_are_errors(gx):
         gx.explore_batch([], [], not_a_field=1)
     with pytest.raises(TypeError):
         gx.Server(slot=4)
+
+
+def test_null_pointers_are_statuses(gx):
+    """null handles and out-pointers give GX_E_ARG (or nothing, for the
+    void defaults functions) instead of crashing the caller"""
+    L = gx.lib()
+    L.gx_options_default(None)
+    L.gx_batch_options_default(None)
+    L.gx_server_options_default(None)
+    assert L.gx_model_load_gpf(None, 0, None) == -7
+    assert L.gx_model_info_get(None, None) == -7
+    assert L.gx_explorer_run(None, None) == -7
+    assert L.gx_explorer_dump(None, None) == -7
+    assert L.gx_device_count(None) == -7
