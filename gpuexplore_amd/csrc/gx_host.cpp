@@ -179,6 +179,7 @@ struct gx_explorer {
   Ctrl last{};       // control block at the end of the last run
   bool ran = false;
   ~gx_explorer() {
+    if (st) (void)hipStreamSynchronize(st);  // no copy may still target h_ctrl
     if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -504,17 +505,22 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   };
   const auto t0 = std::chrono::steady_clock::now();
   int cur = 0;
+  // an error leaves no batch in flight behind it
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  };
   {
     const int rc = enqueue(0, front);
-    if (rc != GX_OK) return rc;
+    if (rc != GX_OK) return fail(rc);
   }
   if (ahead) {
     const int rc = enqueue(1, front);
-    if (rc != GX_OK) return rc;
+    if (rc != GX_OK) return fail(rc);
   }
   for (;;) {
     int rc = collect(cur);
-    if (rc != GX_OK) return rc;
+    if (rc != GX_OK) return fail(rc);
     ch = e->h_ctrl[cur];
     prev_front = front;
     front = static_cast<double>(ch.front_count[ch.cur]);
@@ -531,22 +537,22 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
     }
     if (opt->verbosity == 3) {  // cu:2136-2139
       const int prc = print_states(e, stdout, ch);
-      if (prc != GX_OK) return prc;
+      if (prc != GX_OK) return fail(prc);
     }
     if (ch.stop) {
       if (ahead) {  // the batch behind it found stop set and did nothing
         rc = collect(cur ^ 1);
-        if (rc != GX_OK) return rc;
+        if (rc != GX_OK) return fail(rc);
       }
       break;
     }
     if (ahead) {
       rc = enqueue(cur, ahead_front());
-      if (rc != GX_OK) return rc;
+      if (rc != GX_OK) return fail(rc);
       cur ^= 1;
     } else {
       rc = enqueue(cur, front);
-      if (rc != GX_OK) return rc;
+      if (rc != GX_OK) return fail(rc);
     }
   }
   const auto t1 = std::chrono::steady_clock::now();
