@@ -486,7 +486,11 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   BX_TRY(d_table.alloc(total_buckets * bb));
   hipStream_t st;
   BX_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  std::unique_ptr<void, void (*)(void *)> sg(st, [](void *s) { (void)hipStreamDestroy(static_cast<hipStream_t>(s)); });
+  // on every return, the stream's work ends before the buffers are freed
+  std::unique_ptr<void, void (*)(void *)> sg(st, [](void *s) {
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(s));
+    (void)hipStreamDestroy(static_cast<hipStream_t>(s));
+  });
   BX_TRY(hipMemsetAsync(d_table.p, 0, total_buckets * bb, st));
   BX_TRY(hipMemcpyAsync(d_models.p, mbuf.data(), mbuf.size() * 4, hipMemcpyHostToDevice, st));
   BX_TRY(hipMemcpyAsync(d_inst.p, inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice, st));

@@ -606,7 +606,10 @@ struct gx_table {
   uint64_t seed = 0;
   hipStream_t st = nullptr;
   ~gx_table() {
-    if (st) (void)hipStreamDestroy(st);
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   }
 };
 

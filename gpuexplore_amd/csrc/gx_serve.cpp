@@ -82,6 +82,7 @@ struct gx_server {
   uint64_t steps = 0, rounds = 0;
   uint32_t one = 1;  // host source of the prefix_only flag copy
   ~gx_server() {
+    if (st) (void)hipStreamSynchronize(st);  // nothing may still run on the pools
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (st) (void)hipStreamDestroy(st);
