@@ -272,6 +272,13 @@ def main() -> int:
             "t_hash_s_per_step": t_hash / args.steps,
             "t_expand_s_per_step": sum(r["t_expand_s"] for r in runs) / args.steps,
         },
+        # the hash bytes plus each new state's frontier entry written once and
+        # read once, counted at the reference's 4 * sv_nints bytes per vector
+        # (the frontier here holds 8 * key_words bytes per state)
+        "whole_iteration": {
+            "bytes_per_step": (algo_bytes + 2 * 4 * info["sv_nints"] * inserts) / args.steps,
+            "gbps": (algo_bytes + 2 * 4 * info["sv_nints"] * inserts) / t_exp / 1e9,
+        },
         "cpu_baseline": cpu,
         "batched_vs_plain": batch,
         "por": por,
