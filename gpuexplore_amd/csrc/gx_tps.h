@@ -11,13 +11,16 @@
 //
 // Row data comes from the packed 8-word row records (one 16-byte and one
 // 8-byte read per process) and (tstart, tcnt) pairs per (row, label) entry.
-// The enabled sets of up to kTpsLabCache labels are computed in one pass over
-// the processes and handed from the scan to the emission, together with the
-// lane's enabled rules of each such label as a bit mask: the 64 lanes of a
-// wave hold different states, and walking every rule of a label would make
-// each lane pay for the rules enabled in any lane.  A rule's product is 1
-// unless a member's row has a synchronising entry with a target count other
-// than 1 (kRowMultiSync), so members are visited only then.
+// The enabled sets of up to kTpsLabCache labels are computed in one extra
+// pass over the processes, or in the first pass itself when the whole model
+// has at most kTpsLabCache synchronising labels.  They are handed from the
+// scan to the emission, together with the lane's enabled rules of each such
+// label as a bit mask: the 64 lanes of a wave hold different states, and
+// walking every rule of a label would make each lane pay for the rules
+// enabled in any lane.  A rule's product is 1 unless a member's row has a
+// synchronising entry with a target count other than 1 (kRowMultiSync), so
+// members are visited only then.  Successors are built from the model's
+// key-shaped records: (key & ~field mask) | target record.
 #pragma once
 #include "gx_device.h"
 
