@@ -7,6 +7,7 @@
 // proviso, a 32-bit level stamp appended after bit K), so a key is CASed into
 // a slot as is, and an all-zero word marks an empty slot.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 namespace gx {
@@ -92,6 +93,15 @@ struct Ctrl {
   uint32_t levels_done;     // levels completed
   uint32_t truncated;       // stopped by max_states
   uint64_t max_states;      // stop after the level that reaches this many states (0: no bound)
+  // Partial counters, folded into inserts, lookups, transitions and expanded
+  // by k_advance (or k_fold): a workgroup adds into slot blockIdx % 64, the
+  // slots 128 B apart, instead of every wave of the device hitting one
+  // address at its end.  Kept last: the host copies the fields before it.
+  uint64_t part[64 * 16];  // kCounterSlots x 16
 };
+constexpr uint32_t kCounterSlots = 64;
+// bytes of Ctrl before the partial counters: what the host reads back
+constexpr size_t kCtrlHead = offsetof(Ctrl, part);
+enum CounterIdx : uint32_t { kCntInserts = 0, kCntLookups = 1, kCntTransitions = 2, kCntExpanded = 3 };
 
 }  // namespace gx

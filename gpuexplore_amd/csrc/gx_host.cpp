@@ -476,7 +476,7 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
       b.exp.push_back({a0, a1});
       b.ins.push_back({b0, b1});
     }
-    GX_TRY(hipMemcpyAsync(e->h_ctrl + k, e->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    GX_TRY(hipMemcpyAsync(e->h_ctrl + k, e->d_ctrl.p, kCtrlHead, hipMemcpyDeviceToHost, st));
     b.done = e->evs[k].next();
     if (!b.done) return GX_E_HIP;
     GX_TRY(hipEventRecord(b.done, st));
@@ -667,7 +667,8 @@ static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8
   hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, prop.multiProcessorCount * kInsertBlocksPerCU,
                                 kInsertThreads, t->st);
   GX_TRY(hipEventRecord(e1, t->st));
-  GX_TRY(hipMemcpyAsync(&c, t->ctrl.p, sizeof c, hipMemcpyDeviceToHost, t->st));
+  if (le == hipSuccess) le = launch_fold(ia.ctrl, t->st);
+  GX_TRY(hipMemcpyAsync(&c, t->ctrl.p, kCtrlHead, hipMemcpyDeviceToHost, t->st));
   GX_TRY(hipStreamSynchronize(t->st));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);

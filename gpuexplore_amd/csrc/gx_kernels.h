@@ -159,6 +159,8 @@ hipError_t launch_insert(const InsertArgs &a, int key_words, int slot_words, int
                          bool stamp, uint32_t blocks, uint32_t threads, hipStream_t s);
 
 hipError_t launch_advance(Ctrl *c, hipStream_t s);
+// folds the partial counters into Ctrl's counters (k_advance also does)
+hipError_t launch_fold(Ctrl *c, hipStream_t s);
 
 hipError_t launch_count(const Table &t, int slot_words, int bucket_words, uint64_t *dev_count,
                         uint32_t blocks, hipStream_t s);

@@ -22,6 +22,13 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 // 20 16-state processes); 384 no longer fits eight workgroups per CU in LDS.
 constexpr int kStage = 256;
 
+// adds v to counter `which` through the workgroup's partial-counter slot
+__device__ __forceinline__ void add_counter(Ctrl *C, uint32_t which, uint64_t v) {
+  if (v)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&C->part[(blockIdx.x % kCounterSlots) * 16u + which]),
+              static_cast<unsigned long long>(v));
+}
+
 // ---------------------------------------------------------------------------
 // Successor generation.
 //
@@ -391,10 +398,8 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   }
   ch.release_all<W>();
   if (lane == 0) {
-    if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
-                             static_cast<unsigned long long>(acc_trans));
-    if (acc_states) atomicAdd(reinterpret_cast<unsigned long long *>(&C->expanded),
-                              static_cast<unsigned long long>(acc_states));
+    add_counter(C, kCntTransitions, acc_trans);
+    add_counter(C, kCntExpanded, acc_states);
   }
   if (ballot(viol) && lane == 0) C->violation = 1;
   if (ballot(bad) && lane == 0) C->bad_state = 1;
@@ -479,10 +484,8 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   }
   ch.release_all<W>();
   if (lane == 0) {
-    if (acc_trans) atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions),
-                             static_cast<unsigned long long>(acc_trans));
-    if (acc_states) atomicAdd(reinterpret_cast<unsigned long long *>(&C->expanded),
-                              static_cast<unsigned long long>(acc_states));
+    add_counter(C, kCntTransitions, acc_trans);
+    add_counter(C, kCntExpanded, acc_states);
   }
   if (ballot(viol) && lane == 0) C->violation = 1;
   if (ballot(bad) && lane == 0) C->bad_state = 1;
@@ -620,12 +623,10 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
   uint64_t looks = 0;  // lookups were counted per lane
   (void)wave_excl_scan(acc_look, &looks);
   if (lane == 0) {
-    if (looks) atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(looks));
-    if (acc_new) atomicAdd(reinterpret_cast<unsigned long long *>(&C->inserts), static_cast<unsigned long long>(acc_new));
-    if (acc_trans)
-      atomicAdd(reinterpret_cast<unsigned long long *>(&C->transitions), static_cast<unsigned long long>(acc_trans));
-    if (acc_states)
-      atomicAdd(reinterpret_cast<unsigned long long *>(&C->expanded), static_cast<unsigned long long>(acc_states));
+    add_counter(C, kCntLookups, looks);
+    add_counter(C, kCntInserts, acc_new);
+    add_counter(C, kCntTransitions, acc_trans);
+    add_counter(C, kCntExpanded, acc_states);
   }
   if (ballot(viol) && lane == 0) C->violation = 1;
   if (ballot(bad) && lane == 0) C->bad_state = 1;
@@ -752,10 +753,10 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
     staged += cnt;
   }
   if (a.append) flush();
-  if (lane == 0 && acc_new)
-    atomicAdd(reinterpret_cast<unsigned long long *>(&C->inserts), static_cast<unsigned long long>(acc_new));
-  if (lane == 0 && acc_look)
-    atomicAdd(reinterpret_cast<unsigned long long *>(&C->lookups), static_cast<unsigned long long>(acc_look));
+  if (lane == 0) {
+    add_counter(C, kCntInserts, acc_new);
+    add_counter(C, kCntLookups, acc_look);
+  }
   if (ballot(full) && lane == 0) C->table_full = 1;
   if (ballot(timeout) && lane == 0) C->spin_timeout = 1;
   if (ballot(overflow) && lane == 0) C->overflow = 1;
@@ -766,8 +767,33 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
 // deferred; then the frontiers swap.  Exploration stops after a level with a
 // violation (level-granular early stop), on a full table or a fatal error,
 // or when the next frontier is empty.
+// Folds the partial-counter slots into the counters and clears them.  One
+// wave of kCounterSlots lanes, lane i taking slot i.
+__device__ __forceinline__ void fold_counters(Ctrl *c) {
+  const uint32_t i = threadIdx.x;
+  uint64_t *slot = c->part + i * 16u;
+  uint64_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = slot[k];
+    slot[k] = 0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    for (int d = kCounterSlots / 2; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, kCounterSlots);
+  if (i == 0) {
+    c->inserts += v[kCntInserts];
+    c->lookups += v[kCntLookups];
+    c->transitions += v[kCntTransitions];
+    c->expanded += v[kCntExpanded];
+  }
+}
+
+__global__ void k_fold(Ctrl *c) { fold_counters(c); }
+
 __global__ void k_advance(Ctrl *c) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  fold_counters(c);
+  if (threadIdx.x != 0) return;
   if (c->stop) return;
   c->cand_count = 0;
   if (c->overflow || c->bad_state || c->spin_timeout || c->table_full) {
@@ -935,7 +961,12 @@ hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks
 }
 
 hipError_t launch_advance(Ctrl *c, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, c);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(kCounterSlots), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold(Ctrl *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kCounterSlots), 0, s, c);
   return hipGetLastError();
 }
 
