@@ -5,7 +5,7 @@ then the FETCH_SIZE counter, then WRITE_SIZE; counters are never combined
 with tracing), then summarise them:
 
     python -m gpuexplore_amd.profsum collect --out /abs/dir -- \\
-        python3 /abs/repo/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0
+        python3 $PWD/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0
     python -m gpuexplore_amd.profsum summarize --dir /abs/dir --name k_insert_config2
 
 The counters are scaled by what they read for bench.py's streaming copy
@@ -96,7 +96,7 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
                        "dispatches": fcalls[k], "level_launches": n_level,
                        "bytes_per_launch": (fetch[k] * fscale + write.get(k, 0.0) * wscale) / n_level}
     roof = bench["roofline"]
-    cmd = "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0"
+    cmd = "python3 $PWD/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0"
     data = {"workload": bench["config"]["workload"], "command": f"rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -- {cmd}",
             "fetch_scale": fscale, "write_scale": wscale, "kernels": kern,
             "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
@@ -107,7 +107,8 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
         json.dump(data, f, indent=1)
     md = [f"# {kernel} on {bench['config']['workload']}", "",
           "Kernel time, `rocprofv3 --kernel-trace --stats`, one exploration:", "",
-          "```", f"cd /tmp && python3 -m gpuexplore_amd.profsum collect --out <dir> -- {cmd}",
+          "From the repository root (each profiler pass runs in /tmp with TMPDIR=/tmp; <dir> absolute):", "",
+          "```", f"python3 -m gpuexplore_amd.profsum collect --out <dir> -- {cmd}",
           f"python3 -m gpuexplore_amd.profsum summarize --dir <dir> --name {name}", "```", "",
           "| kernel | calls | total ms | average us |", "|---|---|---|---|"]
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:8]:
