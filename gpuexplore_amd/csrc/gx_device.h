@@ -277,10 +277,14 @@ constexpr int kSpinLimit = 1 << 22;
 // change once written, so a stale (cached) view of a bucket can only show a
 // slot as emptier than it is; a CAS on such a slot fails and returns the
 // real first word, which replaces the stale one.
-template <int WT, int BW>
-__device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t (&v)[WT],
-                                                 const SlotFmt<WT> &f, uint64_t h, int sub,
-                                                 int group_base, bool active) {
+//
+// `b0` is the key's first bucket, bucket_of(h, 0, T.nbuckets); `rehash()`
+// returns h, for the later probes only (callers that computed the first
+// bucket elsewhere need not keep h live).
+template <int WT, int BW, typename Rehash>
+__device__ __forceinline__ int find_or_put_group_b(const Table &T, const uint64_t (&v)[WT],
+                                                   const SlotFmt<WT> &f, uint64_t b0, Rehash rehash,
+                                                   int sub, int group_base, bool active) {
   constexpr int LPB = BW / 2;
   constexpr int SPB = BW / WT;
   static_assert(SPB >= 1, "slot wider than bucket");
@@ -308,7 +312,9 @@ __device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t 
     while (ballot(status == kPending)) {
       if (status == kPending) {
         if (load) {
-          bucket = T.words + bucket_of(h, probe, T.nbuckets) * BW;
+          uint64_t bi = b0;
+          if (probe) bi = bucket_of(rehash(), probe, T.nbuckets);
+          bucket = T.words + bi * BW;
           if (load == 1) {
             const ulonglong2 q = *reinterpret_cast<const ulonglong2 *>(bucket + x0);
             w0 = q.x;
@@ -354,7 +360,9 @@ __device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t 
   while (ballot(status == kPending)) {
     if (status == kPending) {
       if (load) {
-        bucket = T.words + bucket_of(h, probe, T.nbuckets) * BW;
+        uint64_t bi = b0;
+        if (probe) bi = bucket_of(rehash(), probe, T.nbuckets);
+        bucket = T.words + bi * BW;
         if (load == 1) {
           const ulonglong2 q = *reinterpret_cast<const ulonglong2 *>(bucket + x0);
           w0 = q.x;
@@ -429,6 +437,14 @@ __device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t 
     }
   }
   return status;
+}
+
+template <int WT, int BW>
+__device__ __forceinline__ int find_or_put_group(const Table &T, const uint64_t (&v)[WT],
+                                                 const SlotFmt<WT> &f, uint64_t h, int sub,
+                                                 int group_base, bool active) {
+  return find_or_put_group_b<WT, BW>(T, v, f, bucket_of(h, 0, T.nbuckets), [h]() { return h; }, sub,
+                                     group_base, active);
 }
 
 // Thread-level FIND: returns 1 and the slot's stamp if the key is present.

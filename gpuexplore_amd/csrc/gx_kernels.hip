@@ -671,8 +671,13 @@ hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, boo
 // LDS and appended to the next frontier in blocks of up to kStage keys, one
 // atomicAdd per block: a single contended counter would otherwise serialise
 // all waves of the device.
+//
+// The lookups are latency bound, so resident waves matter: eight per SIMD
+// are asked for explicitly (the hashed steps below otherwise take 106 SGPRs,
+// seven waves; asking costs a few SGPR spills to VGPR lanes, and FIND-OR-PUT
+// runs 4 % faster).
 template <int W, int WT, int BW, bool STAMP>
-__global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_insert(InsertArgs a) {
   extern __shared__ uint64_t stage_all[];
   Ctrl *C = a.ctrl;
   if (C->stop) return;
@@ -710,39 +715,22 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
     wave_sync();
     staged = 0;
   };
-  // Each wave walks runs of consecutive iterations.  Successors of one
-  // state, and of states found next to each other, are adjacent in the
-  // candidate buffer, so the run keeps them on one wave: its new keys reach
-  // the next frontier in candidate order (siblings stay together), and the
-  // duplicates that commuting moves produce are looked up close in time.
-  const uint64_t RUN = CPW * insert_run(n, nw * CPW);
-  for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
-    const uint64_t ci = c0 + cg;
-    c0 += CPW;
-    if (c0 == r0 + RUN) {
-      r0 += nw * RUN;
-      c0 = r0;
-    }
-    bool active = ci < n;
-    uint64_t key[W];
-#pragma unroll
-    for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
-      key[j] = active ? __builtin_nontemporal_load(a.cand + ci * W + j) : kFlag;
-    active = active && key[0] != 0;  // released reservation
+  // One iteration: group cg looks up candidate ci (key, first bucket b0).
+  auto visit = [&](uint64_t ci, const uint64_t (&key)[W], uint64_t b0, bool active) {
     uint64_t v[WT];
     slot_value<W, WT>(key, fmt, v);
-    const uint64_t h = key_hash<W>(key, a.table.seed);
-    const int st = find_or_put_group<WT, BW>(a.table, v, fmt, h, sub, gbase, active);
+    const int st = find_or_put_group_b<WT, BW>(
+        a.table, v, fmt, b0, [&]() { return key_hash<W>(key, a.table.seed); }, sub, gbase, active);
     acc_look += __popcll(ballot(active && sub == 0));
     const bool is_new = active && st == kInserted;
     full |= active && st == kFull;
     timeout |= active && st == kTimeout;
     if (a.is_new && active && sub == 0) a.is_new[ci] = is_new ? 1 : 0;
     const uint64_t m = ballot(is_new && sub == 0);
-    if (!m) continue;
+    if (!m) return;
     const uint32_t cnt = __popcll(m);
     acc_new += cnt;
-    if (!a.append) continue;
+    if (!a.append) return;
     if (staged + cnt > static_cast<uint32_t>(kStage)) flush();
     if (is_new) {
       const uint32_t pos = staged + __popcll(m & ((1ull << gbase) - 1ull));
@@ -751,6 +739,59 @@ __global__ __launch_bounds__(512) void k_insert(InsertArgs a) {
         if (j % LPB == sub) stage[pos * W + j] = key[j];
     }
     staged += cnt;
+  };
+  // Each wave walks runs of consecutive iterations.  Successors of one
+  // state, and of states found next to each other, are adjacent in the
+  // candidate buffer, so the run keeps them on one wave: its new keys reach
+  // the next frontier in candidate order (siblings stay together), and the
+  // duplicates that commuting moves produce are looked up close in time.
+  const uint64_t iters = insert_run(n, nw * CPW);
+  if (iters < kInsertSpread) {
+    // a pass spread over every wave: one iteration of CPW candidates at a time
+    const uint64_t RUN = CPW * iters;
+    for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
+      const uint64_t ci = c0 + cg;
+      c0 += CPW;
+      if (c0 == r0 + RUN) {
+        r0 += nw * RUN;
+        c0 = r0;
+      }
+      uint64_t key[W];
+#pragma unroll
+      for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
+        key[j] = ci < n ? __builtin_nontemporal_load(a.cand + ci * W + j) : 0;
+      // a zero first word: past the end or a released reservation
+      visit(ci, key, bucket_of(key_hash<W>(key, a.table.seed), 0, a.table.nbuckets), key[0] != 0);
+    }
+  } else {
+    // Runs of kInsertRun iterations in steps of 64 candidates: each lane
+    // reads and hashes one candidate, and the step's LPB iterations hand
+    // them to the groups in candidate order, so a candidate is hashed once
+    // rather than by every lane of its group.
+    static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
+    const uint64_t RUN = CPW * kInsertRun;
+    for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
+      const uint64_t cl = c0 + lane;
+      const uint64_t cstep = c0;
+      c0 += 64;
+      if (c0 == r0 + RUN) {
+        r0 += nw * RUN;
+        c0 = r0;
+      }
+      uint64_t key_l[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) key_l[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
+      const uint64_t act_m = ballot(key_l[0] != 0);
+      const uint64_t b_l = bucket_of(key_hash<W>(key_l, a.table.seed), 0, a.table.nbuckets);
+#pragma unroll 1
+      for (int s = 0; s < LPB; s++) {
+        const int src = s * CPW + cg;
+        uint64_t key[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) key[j] = __shfl(key_l[j], src, 64);
+        visit(cstep + src, key, __shfl(b_l, src, 64), (act_m >> src) & 1ull);
+      }
+    }
   }
   if (a.append) flush();
   if (lane == 0) {
