@@ -17,7 +17,9 @@
 // scan to the emission, together with the lane's enabled rules of each such
 // label as a bit mask: the 64 lanes of a wave hold different states, and
 // walking every rule of a label would make each lane pay for the rules
-// enabled in any lane.  A rule's product is 1 unless a member's row has a
+// enabled in any lane.  For the same reason each lane walks its own
+// processes with local targets (a bit mask from the scan) instead of the
+// wave stepping through every process.  A rule's product is 1 unless a member's row has a
 // synchronising entry with a target count other than 1 (kRowMultiSync), so
 // members are visited only then.  Successors are built from the model's
 // key-shaped records: (key & ~field mask) | target record.
@@ -93,6 +95,7 @@ __device__ __forceinline__ uint32_t tps_enabled(const uint32_t *M, const ModelHd
 struct TpsInfo {
   uint64_t labs;  // synchronising labels present in the state
   uint64_t cnt;   // successors
+  uint32_t lmask; // processes whose row has local targets
   uint32_t en[kTpsLabCache];     // enabled sets of the first labels of labs (when cached)
   uint64_t rmask[kTpsLabCache];  // enabled rules of those labels, bit k = rule start + k
   bool cached;    // labs has at most kTpsLabCache labels
@@ -153,6 +156,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     rows[p * 64 + lane] = static_cast<RowT>(row);
     if (row != kNoRow) {
       r.cnt += ra.y - ra.x;
+      if (ra.y > ra.x) r.lmask |= 1u << p;
       anyloc |= (ra.w & kRowHasLocal) != 0;
       if (ra.w & kRowMultiSync) multi |= 1u << p;
       const uint64_t rl = tps_rowlab(M, H, row);
@@ -272,21 +276,34 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
                                          const RowT *rows, int lane, const TpsInfo &inf, Emit &&emit) {
   const int N = static_cast<int>(H.nprocs);
   const uint32_t kw = H.key_words;
-  for (int p = 0; p < N; p++) {
-    const uint32_t row = row_at(rows, p * 64 + lane);
-    if (row == kNoRow) continue;
-    const uint4 ra = tps_row(M, H, row);
-    if (ra.x == ra.y) continue;
+  // Local moves: each lane walks its own processes with local targets
+  // (ascending, targets in order), so the wave takes as many steps as its
+  // busiest lane has local successors, not the sum over processes of the
+  // busiest lane per process.
+  {
+    uint32_t lm = inf.lmask, t = 0, te = 0;
     uint64_t base[W];
-    tps_keyrec<W>(M, H.off_fclr, static_cast<uint32_t>(p), kw, base);
 #pragma unroll
-    for (int j = 0; j < W; j++) base[j] = key[j] & ~base[j];
-    for (uint32_t t = ra.x; t < ra.y; t++) {
-      uint64_t s[W];
-      tps_keyrec<W>(M, H.off_locx, t, kw, s);
+    for (int j = 0; j < W; j++) base[j] = 0;
+    while (ballot(lm != 0 || t < te)) {
+      if (t == te && lm) {
+        const uint32_t p = static_cast<uint32_t>(__ffs(lm) - 1);
+        lm &= lm - 1u;
+        const uint4 ra = tps_row(M, H, row_at(rows, static_cast<int>(p) * 64 + lane));
+        t = ra.x;
+        te = ra.y;
+        tps_keyrec<W>(M, H.off_fclr, p, kw, base);
 #pragma unroll
-      for (int j = 0; j < W; j++) s[j] |= base[j];
-      emit(s);
+        for (int j = 0; j < W; j++) base[j] = key[j] & ~base[j];
+      }
+      if (t < te) {
+        uint64_t s[W];
+        tps_keyrec<W>(M, H.off_locx, t, kw, s);
+#pragma unroll
+        for (int j = 0; j < W; j++) s[j] |= base[j];
+        emit(s);
+        t++;
+      }
     }
   }
   int i = 0;
