@@ -715,12 +715,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
     wave_sync();
     staged = 0;
   };
-  // One iteration: group cg looks up candidate ci (key, first bucket b0).
-  auto visit = [&](uint64_t ci, const uint64_t (&key)[W], uint64_t b0, bool active) {
+  // One iteration: group cg looks up candidate ci (key, its hash h and
+  // first bucket b0).
+  auto visit = [&](uint64_t ci, const uint64_t (&key)[W], uint64_t h, uint64_t b0, bool active) {
     uint64_t v[WT];
     slot_value<W, WT>(key, fmt, v);
-    const int st = find_or_put_group_b<WT, BW>(
-        a.table, v, fmt, b0, [&]() { return key_hash<W>(key, a.table.seed); }, sub, gbase, active);
+    const int st = find_or_put_group_b<WT, BW>(a.table, v, fmt, b0, [h]() { return h; }, sub, gbase, active);
     acc_look += __popcll(ballot(active && sub == 0));
     const bool is_new = active && st == kInserted;
     full |= active && st == kFull;
@@ -761,13 +761,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
       for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
         key[j] = ci < n ? __builtin_nontemporal_load(a.cand + ci * W + j) : 0;
       // a zero first word: past the end or a released reservation
-      visit(ci, key, bucket_of(key_hash<W>(key, a.table.seed), 0, a.table.nbuckets), key[0] != 0);
+      const uint64_t h = key_hash<W>(key, a.table.seed);
+      visit(ci, key, h, bucket_of(h, 0, a.table.nbuckets), key[0] != 0);
     }
   } else {
     // Runs of kInsertRun iterations in steps of 64 candidates: each lane
     // reads and hashes one candidate, and the step's LPB iterations hand
-    // them to the groups in candidate order, so a candidate is hashed once
-    // rather than by every lane of its group.
+    // key, hash and first bucket to the groups in candidate order, so a
+    // candidate is hashed once rather than by every lane of its group (the
+    // hash is handed over too: recomputed for the later probes, it would be
+    // hoisted out of the probe loop and computed for every candidate).
     static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
     const uint64_t RUN = CPW * kInsertRun;
     for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
@@ -782,14 +785,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
 #pragma unroll
       for (int j = 0; j < W; j++) key_l[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
       const uint64_t act_m = ballot(key_l[0] != 0);
-      const uint64_t b_l = bucket_of(key_hash<W>(key_l, a.table.seed), 0, a.table.nbuckets);
+      const uint64_t h_l = key_hash<W>(key_l, a.table.seed);
+      const uint64_t b_l = bucket_of(h_l, 0, a.table.nbuckets);
 #pragma unroll 1
       for (int s = 0; s < LPB; s++) {
         const int src = s * CPW + cg;
         uint64_t key[W];
 #pragma unroll
         for (int j = 0; j < W; j++) key[j] = __shfl(key_l[j], src, 64);
-        visit(cstep + src, key, __shfl(b_l, src, 64), (act_m >> src) & 1ull);
+        visit(cstep + src, key, __shfl(h_l, src, 64), __shfl(b_l, src, 64), (act_m >> src) & 1ull);
       }
     }
   }
