@@ -171,6 +171,33 @@ def test_table_vs_set(gpu, W, bb):
     assert len(uniq) >= ins
 
 
+@pytest.mark.parametrize("W,bb", [(2, 64), (1, 128), (3, 128)])
+def test_table_vs_set_runs(gpu, W, bb):
+    """passes large enough that each wave takes runs in steps of 64
+    candidates (a key hashed once and handed to its lane group), with
+    duplicates a few candidates apart: in one step, across groups and across
+    steps; exactly one copy of every distinct key is new"""
+    rng = np.random.default_rng(7 + W)
+    n = 9_000_000  # above 8 iterations of every wave of a 256-CU grid at 64 B buckets
+    base = rng.integers(0, 1 << 62, size=(n // 4, W), dtype=np.uint64)
+    # random 62-bit rows: distinct rows are distinct keys (a collision among
+    # 2.25 M rows has probability below 1e-6), so a key is its row index
+    assert len(np.unique(base[:, 0])) == len(base) or W > 1
+    idx = np.minimum(np.arange(n) // 4 + rng.integers(0, 8, size=n), len(base) - 1)
+    keys = np.ascontiguousarray(base[idx])
+    say("table runs", W, bb, n)
+    t = gpu.Table(1 << 27, W, bb, seed=3 * W)
+    flags, ins, rc = t.insert(keys.ravel())
+    assert rc == 0
+    n_distinct = len(np.unique(idx))
+    assert ins == n_distinct
+    per_key = np.bincount(idx, weights=flags.astype(np.float64), minlength=len(base))
+    assert (per_key[np.unique(idx)] == 1).all()
+    flags2, ins2, rc = t.insert(keys.ravel())
+    assert rc == 0 and ins2 == 0 and not flags2.any()
+    assert t.count() == n_distinct
+
+
 def test_table_full_reports(gpu):
     t = gpu.Table(256, 1, 128)  # 8 buckets x 16 slots
     keys = np.arange(1, 1001, dtype=np.uint64)
