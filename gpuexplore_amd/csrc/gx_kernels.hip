@@ -22,6 +22,27 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 // 20 16-state processes); 384 no longer fits eight workgroups per CU in LDS.
 constexpr int kStage = 256;
 
+// Copies n words of the model into LDS with the whole workgroup, eight
+// loads in flight per thread: a loop with one load per iteration waits a
+// memory round trip per 4 B x blockDim, which made a level of one state
+// take 30-58 us in successor generation.
+__device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *__restrict__ src, uint32_t n) {
+  constexpr uint32_t K = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
+    uint32_t v[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t i = i0 + k * blockDim.x;
+      v[k] = i < n ? src[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t i = i0 + k * blockDim.x;
+      if (i < n) dst[i] = v[k];
+    }
+  }
+}
+
 // adds v to counter `which` through the workgroup's partial-counter slot
 __device__ __forceinline__ void add_counter(Ctrl *C, uint32_t which, uint64_t v) {
   if (v)
@@ -50,7 +71,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   const ModelHdr &H = a.hdr;
   Ctrl *C = a.ctrl;
   if (LDS) {
-    for (uint32_t i = threadIdx.x; i < H.total_words; i += blockDim.x) smem[i] = a.model[i];
+    stage_words(smem, a.model, H.total_words);
   }
   const uint32_t *M = LDS ? smem : a.model;
   const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
@@ -420,7 +441,7 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   Ctrl *C = a.ctrl;
   const uint32_t sec = a.hdr.total_words - a.hdr.tps_begin;
   if (LDS) {
-    for (uint32_t i = threadIdx.x; i < sec; i += blockDim.x) smem[i] = a.model[a.hdr.tps_begin + i];
+    stage_words(smem, a.model + a.hdr.tps_begin, sec);
   }
   const uint32_t *M = LDS ? smem : a.model;
   const uint32_t xoff = LDS ? ((sec + 3u) & ~3u) : 0u;
@@ -510,7 +531,7 @@ __global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
   Ctrl *C = a.ctrl;
   const uint32_t sec = a.hdr.total_words - a.hdr.tps_begin;
   if (LDS) {
-    for (uint32_t i = threadIdx.x; i < sec; i += blockDim.x) smem[i] = a.model[a.hdr.tps_begin + i];
+    stage_words(smem, a.model + a.hdr.tps_begin, sec);
   }
   const uint32_t *M = LDS ? smem : a.model;
   const uint32_t xoff = LDS ? ((sec + 3u) & ~3u) : 0u;
