@@ -64,6 +64,10 @@ struct ModelHdr {
 
 // kRowMultiSync: a synchronising entry of the row has a target count other than 1
 enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u, kRowMultiSync = 4u };
+// In the packed row records' flag word only: bit kRowLabShift + k is set
+// when the row has an entry for the model's k-th synchronising label (k < 4,
+// labels in ascending order).
+constexpr uint32_t kRowLabShift = 8;
 
 // Exploration modes of the successor kernel.
 enum ExpandMode : int { kModePlain = 0, kModePor = 1, kModePorProviso = 2 };

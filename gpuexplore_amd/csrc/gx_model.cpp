@@ -255,9 +255,21 @@ int compile_model(const Gpf &g, Compiled *out) {
   std::vector<uint32_t> rowrec, synrec;
   const uint32_t nrec = static_cast<uint32_t>(rowflags.size());
   rowrec.reserve(8u * nrec);
+  uint32_t first_labs[4];  // the model's first four synchronising labels
+  {
+    uint64_t L = synlab_all;
+    for (int k = 0; k < 4; k++) {
+      first_labs[k] = L ? static_cast<uint32_t>(__builtin_ctzll(L)) : 64u;
+      L &= L ? L - 1 : 0;
+    }
+  }
   for (uint32_t r = 0; r < nrec; r++) {
-    rowrec.insert(rowrec.end(), {loc_start[r], loc_start[r + 1], syn_start[r], rowflags[r], rowlab[2 * r],
-                                 rowlab[2 * r + 1], 0u, 0u});
+    const uint64_t rl = static_cast<uint64_t>(rowlab[2 * r]) | (static_cast<uint64_t>(rowlab[2 * r + 1]) << 32);
+    uint32_t labbits = 0;
+    for (int k = 0; k < 4; k++)
+      if (first_labs[k] < 64u && ((rl >> first_labs[k]) & 1u)) labbits |= 1u << k;
+    rowrec.insert(rowrec.end(), {loc_start[r], loc_start[r + 1], syn_start[r], rowflags[r] | (labbits << kRowLabShift),
+                                 rowlab[2 * r], rowlab[2 * r + 1], 0u, 0u});
   }
   for (size_t e = 0; e < syn_tstart.size(); e++) {
     synrec.push_back(syn_tstart[e]);

@@ -127,8 +127,10 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   bool anyloc = false, anyen = false;
   uint32_t multi = 0;  // processes whose row has a multi-target synchronising entry
   // A model with at most kTpsLabCache synchronising labels gets every
-  // label's enabled set in this pass (men[k] for its k-th label), so the
-  // processes' label masks are read once.
+  // label's enabled set in this pass (men[k] for its k-th label), from the
+  // label bits of the row records' flag words: the rows' 64-bit label masks
+  // are not read at all.
+  static_assert(kTpsLabCache == 4, "row records carry four label bits");
   const uint64_t msyn = static_cast<uint64_t>(H.synlab_lo) | (static_cast<uint64_t>(H.synlab_hi) << 32);
   const bool men_all = __popcll(msyn) <= kTpsLabCache;
   uint32_t mlab[kTpsLabCache], men[kTpsLabCache];
@@ -159,17 +161,21 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
       if (ra.y > ra.x) r.lmask |= 1u << p;
       anyloc |= (ra.w & kRowHasLocal) != 0;
       if (ra.w & kRowMultiSync) multi |= 1u << p;
-      const uint64_t rl = tps_rowlab(M, H, row);
-      r.labs |= rl;
-      if (men_all) {
+      if (men_all) {  // the row's labels among the model's (at most four) from its flag word
 #pragma unroll
-        for (int k = 0; k < kTpsLabCache; k++)
-          if (mlab[k] < 64u && ((rl >> mlab[k]) & 1ull)) men[k] |= 1u << p;
+        for (int k = 0; k < kTpsLabCache; k++) men[k] |= ((ra.w >> (kRowLabShift + k)) & 1u) << p;
+      } else {
+        r.labs |= tps_rowlab(M, H, row);
       }
       if (safety && p == N - 1)
         for (uint32_t t = ra.x; t < ra.y; t++)
           if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) r.viol = true;
     }
+  }
+  if (men_all) {  // every synchronising label of the model is one of mlab[]
+#pragma unroll
+    for (int k = 0; k < kTpsLabCache; k++)
+      if (mlab[k] < 64u && men[k]) r.labs |= 1ull << mlab[k];
   }
   // enabled sets of the first labels in one pass over the processes
   uint32_t lab[kTpsLabCache];
