@@ -818,7 +818,27 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k
       }
     }
   }
-  if (a.append) flush();
+  if (a.append) {
+    // The waves' last stages share one reservation per workgroup: one
+    // same-address atomic instead of one per wave.  The barriers cost no
+    // occupancy, since a workgroup's LDS is held until its last wave ends.
+    __shared__ unsigned long long wg_count, wg_base;
+    if (threadIdx.x == 0) wg_count = 0;
+    __syncthreads();
+    unsigned long long off = 0;
+    if (lane == 0 && staged) off = atomicAdd(&wg_count, static_cast<unsigned long long>(staged));
+    off = __shfl(off, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) wg_base = wg_count ? atomicAdd(out_count, wg_count) : 0ull;
+    __syncthreads();
+    const uint64_t ob = wg_base + off;
+    for (uint32_t i = lane; i < staged * W; i += 64) {
+      if (ob + i / W < a.front_cap)
+        out[ob * W + i] = stage[i];
+      else
+        overflow = true;
+    }
+  }
   if (lane == 0) {
     add_counter(C, kCntInserts, acc_new);
     add_counter(C, kCntLookups, acc_look);
