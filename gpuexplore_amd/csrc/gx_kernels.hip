@@ -694,11 +694,16 @@ hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, boo
 // all waves of the device.
 //
 // The lookups are latency bound, so resident waves matter: eight per SIMD
-// are asked for explicitly (the hashed steps below otherwise take 106 SGPRs,
-// seven waves; asking costs a few SGPR spills to VGPR lanes, and FIND-OR-PUT
-// runs 4 % faster).
+// are asked for explicitly where slots are at most two words and carry no
+// level stamp (the hashed steps below otherwise take 106 SGPRs, seven waves;
+// asking costs a few SGPR spills to VGPR lanes, and FIND-OR-PUT runs 4 %
+// faster).  Wider slots would spill VGPRs to scratch, so they ask nothing.
+template <int WT, bool STAMP>
+constexpr int kInsertMinWaves = (WT <= 2 && !STAMP) ? 8 : 1;
+
 template <int W, int WT, int BW, bool STAMP>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void k_insert(InsertArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<WT, STAMP>)))
+void k_insert(InsertArgs a) {
   extern __shared__ uint64_t stage_all[];
   Ctrl *C = a.ctrl;
   if (C->stop) return;
