@@ -171,7 +171,7 @@ def test_table_vs_set(gpu, W, bb):
     assert len(uniq) >= ins
 
 
-@pytest.mark.parametrize("W,bb", [(2, 64), (1, 128), (3, 128)])
+@pytest.mark.parametrize("W,bb", [(2, 64), (1, 128), (3, 128), (5, 128)])
 def test_table_vs_set_runs(gpu, W, bb):
     """passes large enough that each wave takes runs in steps of 64
     candidates (a key hashed once and handed to its lane group), with
