@@ -393,6 +393,23 @@ class Table:
             flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ins))
         return flags[:n], ins.value, rc
 
+    def insert_device(self, keys: Sequence[int]):
+        """gx_table_insert_device on a device copy of the keys (the copy is
+        made and freed here).  Returns (inserted, status, kernel ms)."""
+        import numpy as np
+        a = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+        n = a.size // self.key_words
+        d = ctypes.c_void_p()
+        _check(lib().gx_device_alloc(max(8, a.nbytes), ctypes.byref(d)), "device alloc")
+        try:
+            _check(lib().gx_copy_to_device(d, a.ctypes.data_as(ctypes.c_void_p), a.nbytes), "copy")
+            ins = ctypes.c_uint64(0)
+            ms = ctypes.c_float(0)
+            rc = lib().gx_table_insert_device(self.h, d, n, ctypes.byref(ins), ctypes.byref(ms))
+        finally:
+            lib().gx_device_free(d)
+        return ins.value, rc, ms.value
+
     def count(self) -> int:
         c = ctypes.c_uint64(0)
         _check(lib().gx_table_count(self.h, ctypes.byref(c)), "count")

@@ -20,6 +20,10 @@ struct BatchInst {
   uint64_t chunk_prefix;  // first chunk of this instance in the round
   uint32_t cur, level;
   uint32_t violation, table_full, overflow, bad_state, spin_timeout, stop;
+  uint32_t cand_overflow;  // a 64-state chunk of this instance emits more successors than the
+                           // whole candidate buffer holds: the instance stops (more visited-set
+                           // pages cannot help it)
+  uint32_t pad;
 };
 
 struct BatchCtrl {
@@ -32,7 +36,7 @@ struct BatchCtrl {
   uint32_t active;       // instances not stopped
   uint32_t rounds;
   uint32_t n;
-  uint32_t cand_overflow;
+  uint32_t cand_overflow;  // some instance stopped with a chunk larger than the candidate buffer
   uint32_t prefix_only;  // next advance only recomputes the chunk prefix
   uint32_t pad;
 };

@@ -86,9 +86,17 @@ __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
     const uint64_t off = wave_excl_scan(inf.cnt, &total);
     uint64_t obase = 0;
     const int ok = ch.reserve<W>(total, (nchunks - c + nw - 1) / nw, &obase);
-    if (ok <= 0) {
+    if (ok < 0) {
+      // this chunk can never fit: stop its instance (at the level's end),
+      // not the batch, and do not defer the chunk
       if (lane == 0) {
-        if (ok < 0) C->cand_overflow = 1;
+        I->cand_overflow = 1;
+        C->cand_overflow = 1;
+      }
+      continue;
+    }
+    if (ok == 0) {
+      if (lane == 0) {
         const uint64_t at = atomicAdd(reinterpret_cast<unsigned long long *>(def_out_count), 1ull);
         def_out[at] = (static_cast<uint64_t>(i) << 40) | start;
       }
@@ -214,13 +222,8 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   if (C->active == 0 && C->prefix_only == 0) return;
   const uint32_t tid = threadIdx.x;
   const bool deferred = C->deferred_count[C->deferred_cur ^ 1u] > 0;
-  const bool fatal = C->cand_overflow != 0;  // a chunk that can never fit: stop everything
   const bool prefix_only = C->prefix_only != 0;  // recompute chunks after admissions
   __syncthreads();
-  if (fatal) {
-    if (tid == 0) C->active = 0;
-    return;
-  }
   if (tid == 0) {
     C->cand_count = 0;
     if (!prefix_only) C->rounds += 1;
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
         I->front_count[cur] = 0;
         I->cur = nxt;
         I->level += 1u;
-        if (I->violation || I->table_full || I->overflow || I->bad_state || I->spin_timeout) I->stop = 1;
+        if (I->violation || I->table_full || I->overflow || I->bad_state || I->spin_timeout || I->cand_overflow)
+          I->stop = 1;
         if (I->front_count[nxt] == 0) I->stop = 1;
         if (!I->stop) chunks = (I->front_count[nxt] + 63) / 64;
       }
@@ -574,9 +578,8 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
     r.t_explore_s = t_explore;
     r.t_init_s = t_init;
     if (I.bad_state) rc = GX_E_STATE;
-    if ((I.overflow || I.spin_timeout) && rc == GX_OK) rc = GX_E_CAPACITY;
+    if ((I.overflow || I.spin_timeout || I.cand_overflow) && rc == GX_OK) rc = GX_E_CAPACITY;
   }
-  if (ch.cand_overflow && rc == GX_OK) rc = GX_E_CAPACITY;
   if (stats) {
     stats->t_explore_s = t_explore;
     stats->t_init_s = t_init;

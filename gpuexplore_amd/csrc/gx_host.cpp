@@ -603,6 +603,8 @@ struct gx_table {
   DevBuf words, ctrl, cand, status;
   uint64_t nbuckets = 0, cand_cap = 0;
   uint32_t W = 1, BW = 16;
+  uint32_t cus = 1;       // compute units of the table's device
+  int dev = 0;            // the device the table lives on
   uint64_t seed = 0;
   hipStream_t st = nullptr;
   ~gx_table() {
@@ -622,60 +624,68 @@ extern "C" int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t buc
   if (device_count() == 0) return GX_E_NODEVICE;
   gx_table *t = new (std::nothrow) gx_table();
   if (!t) return GX_E_NOMEM;
+  std::unique_ptr<gx_table> guard(t);
   t->W = key_words;
   t->BW = bucket_bytes / 8;
   t->seed = hash_seed;
   t->nbuckets = q_ints * 4ull / bucket_bytes;
-  int rc = GX_OK;
-  if (t->nbuckets == 0) rc = GX_E_ARG;
-  if (rc == GX_OK && hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess) rc = GX_E_HIP;
-  if (rc == GX_OK && t->words.alloc(t->nbuckets * bucket_bytes) != hipSuccess) rc = GX_E_NOMEM;
-  if (rc == GX_OK && t->ctrl.alloc(sizeof(Ctrl)) != hipSuccess) rc = GX_E_NOMEM;
-  if (rc == GX_OK && hipMemsetAsync(t->words.p, 0, t->nbuckets * bucket_bytes, t->st) != hipSuccess) rc = GX_E_HIP;
-  if (rc == GX_OK && hipStreamSynchronize(t->st) != hipSuccess) rc = GX_E_HIP;
-  if (rc != GX_OK) {
-    delete t;
-    return rc;
-  }
-  *out = t;
+  if (t->nbuckets == 0) return GX_E_ARG;
+  GX_TRY(hipGetDevice(&t->dev));
+  hipDeviceProp_t prop;
+  GX_TRY(hipGetDeviceProperties(&prop, t->dev));
+  t->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+  if (hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess) return GX_E_HIP;
+  if (t->words.alloc(t->nbuckets * bucket_bytes) != hipSuccess) return GX_E_NOMEM;
+  if (t->ctrl.alloc(sizeof(Ctrl)) != hipSuccess) return GX_E_NOMEM;
+  GX_TRY(hipMemsetAsync(t->words.p, 0, t->nbuckets * bucket_bytes, t->st));
+  GX_TRY(hipStreamSynchronize(t->st));
+  *out = guard.release();
   return GX_OK;
 }
 
 extern "C" void gx_table_free(gx_table *t) { delete t; }
 
-static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8_t *dev_status,
-                        uint64_t *inserted, float *kernel_ms) {
-  Ctrl c{};
-  c.cand_count = n;
-  GX_TRY(hipMemcpyAsync(t->ctrl.p, &c, sizeof c, hipMemcpyHostToDevice, t->st));
+// One FIND-OR-PUT pass of n keys (device memory) with control block `ctrl`,
+// is-new bytes to dev_status (optional).  set_flag: the keys come from the
+// caller, so bit 63 of every word is set on the device.  kernel_ms: timed
+// with HIP events when non-NULL (events are created only then).
+static int table_insert(gx_table *t, Ctrl *ctrl, const uint64_t *dev_keys, uint64_t n, uint8_t *dev_status,
+                        bool set_flag, uint64_t *inserted, float *kernel_ms) {
   InsertArgs ia{};
-  ia.ctrl = t->ctrl.as<Ctrl>();
+  ia.ctrl = ctrl;
   ia.cand = dev_keys;
   ia.cand_cap = n;
   ia.table = Table{t->words.as<uint64_t>(), t->nbuckets, t->seed};
   ia.state_bits = 63 * t->W;
   ia.is_new = dev_status;
   ia.append = 0;
-  int dev = 0;
-  GX_TRY(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  GX_TRY(hipGetDeviceProperties(&prop, dev));
-  hipEvent_t e0, e1;
-  GX_TRY(hipEventCreate(&e0));
-  GX_TRY(hipEventCreate(&e1));
-  GX_TRY(hipEventRecord(e0, t->st));
-  hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, prop.multiProcessorCount * kInsertBlocksPerCU,
-                                kInsertThreads, t->st);
-  GX_TRY(hipEventRecord(e1, t->st));
-  if (le == hipSuccess) le = launch_fold(ia.ctrl, t->st);
-  GX_TRY(hipMemcpyAsync(&c, t->ctrl.p, kCtrlHead, hipMemcpyDeviceToHost, t->st));
-  GX_TRY(hipStreamSynchronize(t->st));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  ia.set_flag = set_flag ? 1 : 0;
+  GX_TRY(launch_ctrl_reset(ctrl, n, t->st));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (kernel_ms) {
+    GX_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return GX_E_HIP;
+    }
+    (void)hipEventRecord(e0, t->st);
+  }
+  hipError_t le = launch_insert(ia, t->W, t->W, t->BW, false, t->cus * kInsertBlocksPerCU, kInsertThreads, t->st);
+  if (kernel_ms) (void)hipEventRecord(e1, t->st);
+  if (le == hipSuccess) le = launch_fold(ctrl, t->st);
+  Ctrl c{};
+  hipError_t ce = hipMemcpyAsync(&c, ctrl, kCtrlHead, hipMemcpyDeviceToHost, t->st);
+  hipError_t se = hipStreamSynchronize(t->st);
+  if (kernel_ms) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *kernel_ms = ms;
+  }
   GX_TRY(le);
-  if (kernel_ms) *kernel_ms = ms;
+  GX_TRY(ce);
+  GX_TRY(se);
   if (inserted) *inserted = c.inserts;
   if (c.spin_timeout) return GX_E_HIP;
   if (c.table_full) return GX_E_CAPACITY;
@@ -685,7 +695,7 @@ static int table_insert(gx_table *t, const uint64_t *dev_keys, uint64_t n, uint8
 extern "C" int gx_table_insert_device(gx_table *t, const uint64_t *dev_keys, uint64_t n,
                                       uint64_t *inserted, float *kernel_ms) {
   if (!t || (!dev_keys && n)) return GX_E_ARG;
-  return table_insert(t, dev_keys, n, nullptr, inserted, kernel_ms);
+  return table_insert(t, t->ctrl.as<Ctrl>(), dev_keys, n, nullptr, true, inserted, kernel_ms);
 }
 
 extern "C" int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
@@ -697,7 +707,7 @@ extern "C" int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t 
   std::vector<uint64_t> tmp(keys, keys + n * t->W);
   for (auto &w : tmp) w |= kFlag;
   GX_TRY(hipMemcpyAsync(k.p, tmp.data(), n * t->W * 8, hipMemcpyHostToDevice, t->st));
-  int rc = table_insert(t, k.as<uint64_t>(), n, s.as<uint8_t>(), inserted, nullptr);
+  int rc = table_insert(t, t->ctrl.as<Ctrl>(), k.as<uint64_t>(), n, s.as<uint8_t>(), false, inserted, nullptr);
   if (is_new && n) {
     GX_TRY(hipMemcpyAsync(is_new, s.p, n, hipMemcpyDeviceToHost, t->st));
     GX_TRY(hipStreamSynchronize(t->st));

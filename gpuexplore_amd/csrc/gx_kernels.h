@@ -63,7 +63,14 @@ struct InsertArgs {
   uint32_t state_bits;
   uint8_t *is_new;  // optional: 1 per candidate that was inserted
   int append;       // append new keys to front[cur ^ 1]
+  int set_flag;     // keys come from a caller: set bit 63 of every word (a caller's
+                    // zero word would otherwise read as a released or unpublished one)
 };
+
+// Resets a control block for a FIND-OR-PUT pass of n candidates with a
+// kernel (no host copy, no memset): what a caller that cannot allocate or
+// copy from pageable memory needs.
+hipError_t launch_ctrl_reset(Ctrl *c, uint64_t n, hipStream_t s);
 
 // FIND-OR-PUT launch geometry: 256-thread workgroups, 64 per CU by default
 constexpr uint32_t kInsertThreads = 256;

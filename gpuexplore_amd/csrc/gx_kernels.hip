@@ -782,10 +782,11 @@ void k_insert(InsertArgs a) {
         r0 += nw * RUN;
         c0 = r0;
       }
+      const uint64_t fl = a.set_flag ? kFlag : 0ull;
       uint64_t key[W];
 #pragma unroll
       for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
-        key[j] = ci < n ? __builtin_nontemporal_load(a.cand + ci * W + j) : 0;
+        key[j] = ci < n ? (__builtin_nontemporal_load(a.cand + ci * W + j) | fl) : 0;
       // a zero first word: past the end or a released reservation
       const uint64_t h = key_hash<W>(key, a.table.seed);
       visit(ci, key, h, bucket_of(h, 0, a.table.nbuckets), key[0] != 0);
@@ -807,9 +808,10 @@ void k_insert(InsertArgs a) {
         r0 += nw * RUN;
         c0 = r0;
       }
+      const uint64_t fl = a.set_flag ? kFlag : 0ull;
       uint64_t key_l[W];
 #pragma unroll
-      for (int j = 0; j < W; j++) key_l[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
+      for (int j = 0; j < W; j++) key_l[j] = cl < n ? (__builtin_nontemporal_load(a.cand + cl * W + j) | fl) : 0;
       const uint64_t act_m = ballot(key_l[0] != 0);
       const uint64_t h_l = key_hash<W>(key_l, a.table.seed);
       const uint64_t b_l = bucket_of(h_l, 0, a.table.nbuckets);
@@ -881,6 +883,15 @@ __device__ __forceinline__ void fold_counters(Ctrl *c) {
 }
 
 __global__ void k_fold(Ctrl *c) { fold_counters(c); }
+
+// zeroes the control block (head and partial counters) and sets cand_count
+__global__ void k_ctrl_reset(Ctrl *c, uint64_t n) {
+  uint64_t *w = reinterpret_cast<uint64_t *>(c);
+  constexpr uint32_t nw = sizeof(Ctrl) / 8;
+  for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) w[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) c->cand_count = n;
+}
 
 __global__ void k_advance(Ctrl *c) {
   fold_counters(c);
@@ -1053,6 +1064,12 @@ hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks
 
 hipError_t launch_advance(Ctrl *c, hipStream_t s) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(kCounterSlots), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_ctrl_reset(Ctrl *c, uint64_t n, hipStream_t s) {
+  static_assert(sizeof(Ctrl) % 8 == 0, "control block of whole words");
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(256), 0, s, c, n);
   return hipGetLastError();
 }
 

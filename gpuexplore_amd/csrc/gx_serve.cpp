@@ -313,7 +313,7 @@ extern "C" int gx_server_step(gx_server *s, gx_server_status *status) {
     Job &j = s->running[i];
     for (uint64_t q = 0; q < s->slot_npages[i]; q++) s->page_used[s->slot_page0[i] + q] = 0;
     s->slot_used[i] = 0;
-    if ((I.table_full || I.overflow) && j.pages * 2 <= s->npages) {
+    if ((I.table_full || I.overflow) && !I.cand_overflow && j.pages * 2 <= s->npages) {
       // restart with a larger partition: 4x (2x at the pool's limit); a
       // full partition usually means the estimate was off by a large factor
       j.pages = j.pages * 4 <= s->npages ? j.pages * 4 : j.pages * 2;
@@ -335,7 +335,10 @@ extern "C" int gx_server_step(gx_server *s, gx_server_status *status) {
     r.t_admit = s->slot_admit[i];
     r.t_done = t;
     r.restarts = j.restarts;
-    r.status = I.bad_state ? GX_E_STATE : (I.spin_timeout ? GX_E_HIP : GX_OK);
+    r.status = I.bad_state ? GX_E_STATE
+               : I.spin_timeout ? GX_E_HIP
+               : I.cand_overflow ? GX_E_CAPACITY  // more pages cannot fix it: no restart
+                                 : GX_OK;
     s->done.push_back(r);
   }
   s->rounds = ch.rounds;
@@ -350,7 +353,7 @@ extern "C" int gx_server_step(gx_server *s, gx_server_status *status) {
     status->pages_used = used;
     status->pages_total = s->npages;
   }
-  return ch.cand_overflow ? GX_E_CAPACITY : GX_OK;
+  return GX_OK;
 }
 
 extern "C" int gx_server_poll(gx_server *s, gx_job_result *out, int max, int *n) {

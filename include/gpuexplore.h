@@ -249,7 +249,9 @@ int gx_explore_batch(const gx_model *const *models, const int32_t *properties,
 /* ------------------------------------------------------------------ */
 /* Visited-set access (FIND-OR-PUT on its own), the hash-insert phase of the
  * exploration (cu:361-450 FINDORPUT_WARP, cu:656-704 init/count).  Keys are
- * `key_words` 64-bit words whose bit 63 is reserved (set by the library). */
+ * `key_words` 64-bit words whose bit 63 is reserved: the library sets it on
+ * every word, so keys differing only in bit 63 are the same key and words of
+ * zero are valid. */
 typedef struct gx_table gx_table;
 
 int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t bucket_bytes,
@@ -263,8 +265,9 @@ void gx_table_free(gx_table *t);
 int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
                          uint64_t *inserted);
 
-/* Same with keys already resident in device memory (a device pointer);
- * timed with HIP events into *kernel_ms when non-NULL. */
+/* Same with keys already resident in device memory (a device pointer; read
+ * only, bit 63 is set on the fly); timed with HIP events into *kernel_ms when
+ * non-NULL. */
 int gx_table_insert_device(gx_table *t, const uint64_t *dev_keys, uint64_t n,
                            uint64_t *inserted, float *kernel_ms);
 

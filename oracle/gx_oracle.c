@@ -513,12 +513,17 @@ uint64_t orc_por_cluster(const orc_model *m, const uint32_t *state) {
 }
 
 /* ------------------------------------------------------------------ */
-/* visited set: open addressing, key = W words, value = level + 1 (0 = empty) */
+/* visited set: open addressing, key = W words.  With levels, lvl[slot] =
+ * level + 1 (0 = empty); the cycle proviso needs them.  Without (plain
+ * exploration and POR without the proviso) a bitmap marks occupied slots:
+ * 4 * W bytes and one bit per slot, so a presized set for the 945 M states of
+ * the 24-process POR workload explored unreduced fits in host memory. */
 typedef struct {
   uint32_t W;
   uint64_t cap, n, mask, seed;
   uint32_t *keys;
-  uint32_t *lvl;
+  uint32_t *lvl; /* NULL: level-free, occ marks the slots */
+  uint64_t *occ;
 } vset;
 
 static inline uint64_t mix64(uint64_t x) {
@@ -536,27 +541,51 @@ static inline uint64_t vhash(const vset *s, const uint32_t *k) {
   return h;
 }
 
-static int vset_init(vset *s, uint32_t W, uint64_t cap, uint64_t seed) {
+/* capacity: a power of two holding `expect` keys at load <= 1/2 */
+static uint64_t vset_cap_for(uint64_t expect, uint64_t min_cap) {
+  uint64_t c = min_cap;
+  while (c < 2 * expect) c <<= 1;
+  return c;
+}
+
+static int vset_init(vset *s, uint32_t W, uint64_t cap, uint64_t seed, int levels) {
   s->W = W;
   s->cap = cap;
   s->mask = cap - 1;
   s->n = 0;
   s->seed = seed;
   s->keys = (uint32_t *)malloc(cap * W * sizeof(uint32_t));
-  s->lvl = (uint32_t *)calloc(cap, sizeof(uint32_t));
-  return (s->keys && s->lvl) ? ORC_OK : ORC_ERR_NOMEM;
+  s->lvl = levels ? (uint32_t *)calloc(cap, sizeof(uint32_t)) : NULL;
+  s->occ = levels ? NULL : (uint64_t *)calloc((cap + 63) / 64, sizeof(uint64_t));
+  return (s->keys && (s->lvl || s->occ)) ? ORC_OK : ORC_ERR_NOMEM;
 }
 
 static void vset_free(vset *s) {
   free(s->keys);
   free(s->lvl);
+  free(s->occ);
 }
 
-/* returns stored level+1 if present, 0 if absent; *slot gets the slot */
+/* stored level + 1 of slot i (1 for an occupied slot of a level-free set),
+ * 0 if empty */
+static inline uint32_t vset_at(const vset *s, uint64_t i) {
+  if (s->lvl) return s->lvl[i];
+  return (uint32_t)((s->occ[i >> 6] >> (i & 63)) & 1u);
+}
+
+static inline void vset_mark(vset *s, uint64_t i, uint32_t level_plus_1) {
+  if (s->lvl)
+    s->lvl[i] = level_plus_1;
+  else
+    s->occ[i >> 6] |= 1ull << (i & 63);
+}
+
+/* returns stored level+1 if present (1 without levels), 0 if absent; *slot
+ * gets the slot */
 static uint32_t vset_probe(const vset *s, const uint32_t *k, uint64_t *slot) {
   uint64_t i = vhash(s, k) & s->mask;
   for (;;) {
-    uint32_t l = s->lvl[i];
+    uint32_t l = vset_at(s, i);
     if (l == 0) {
       *slot = i;
       return 0;
@@ -571,14 +600,18 @@ static uint32_t vset_probe(const vset *s, const uint32_t *k, uint64_t *slot) {
 
 static int vset_grow(vset *s) {
   vset t;
-  int rc = vset_init(&t, s->W, s->cap * 2, s->seed);
-  if (rc) return rc;
+  int rc = vset_init(&t, s->W, s->cap * 2, s->seed, s->lvl != NULL);
+  if (rc) {
+    vset_free(&t);
+    return rc;
+  }
   for (uint64_t i = 0; i < s->cap; i++) {
-    if (!s->lvl[i]) continue;
+    const uint32_t l = vset_at(s, i);
+    if (!l) continue;
     uint64_t sl;
     vset_probe(&t, s->keys + i * s->W, &sl);
     memcpy(t.keys + sl * s->W, s->keys + i * s->W, s->W * sizeof(uint32_t));
-    t.lvl[sl] = s->lvl[i];
+    vset_mark(&t, sl, l);
     t.n++;
   }
   vset_free(s);
@@ -595,7 +628,7 @@ static int vset_put(vset *s, const uint32_t *k, uint32_t level) {
   uint64_t sl;
   if (vset_probe(s, k, &sl)) return 0;
   memcpy(s->keys + sl * s->W, k, s->W * sizeof(uint32_t));
-  s->lvl[sl] = level + 1;
+  vset_mark(s, sl, level + 1);
   s->n++;
   return 1;
 }
@@ -689,8 +722,13 @@ int orc_explore(const orc_model *m, const orc_options *o, orc_result *res) {
   uint32_t W = m->words;
   vset set;
   vlist cur = {NULL, 0, 0, W}, nxt = {NULL, 0, 0, W};
-  int rc = vset_init(&set, W, 1u << 16, o->hash_seed);
-  if (rc) return rc;
+  /* levels only for the cycle proviso; presized from o->expected_states */
+  int rc = vset_init(&set, W, vset_cap_for(o->expected_states, 1u << 16), o->hash_seed,
+                     o->por && o->cycle_proviso);
+  if (rc) {
+    vset_free(&set);
+    return rc;
+  }
   uint32_t s0[ORC_MAX_WORDS] = {0}; /* all processes in state 0 (cu:673-676) */
   vset_put(&set, s0, 0);
   vlist_push(&cur, s0);
@@ -872,7 +910,8 @@ int orc_explore_mt(const orc_model *m, const orc_options *o, int nthreads, orc_r
     goto out;
   }
   for (uint32_t i = 0; i < MT_SHARDS; i++) {
-    rc = vset_init(&sh.shards[i].set, W, 64, o->hash_seed);
+    /* level-free shards, presized from o->expected_states */
+    rc = vset_init(&sh.shards[i].set, W, vset_cap_for(o->expected_states / MT_SHARDS, 64), o->hash_seed, 0);
     if (rc) goto out;
     pthread_mutex_init(&sh.shards[i].mu, NULL);
   }

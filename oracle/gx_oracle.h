@@ -89,6 +89,7 @@ typedef struct orc_options {
   uint64_t max_states; /* 0 = unbounded; else stop after the level that crosses it */
   uint64_t hash_seed;
   int por_method;     /* ORC_POR_*, used when por is set */
+  uint64_t expected_states; /* presizes the visited set (0: grow from a small table) */
 } orc_options;
 
 typedef struct orc_result {

@@ -3,8 +3,10 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Usage:
  *   gx_oracle_cli <model-without-.gpf> [-d|-p] [--por] [--cycle-proviso]
- *                 [--max-states N] [--seed S]
+ *                 [--max-states N] [--seed S] [--threads T] [--expect N]
  * Prints one key=value line: states, transitions, levels, violation, seconds.
+ * --threads T (plain method only) runs orc_explore_mt on T threads; --expect N
+ * presizes the visited set for N states.
  * Unlike the reference CLI (cu:1851-1874) the options are orthogonal here.
  */
 #include <stdio.h>
@@ -15,11 +17,12 @@
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s <model> [-d|-p] [--por] [--por-method cample|ample|stubborn] [--cycle-proviso] [--max-states N]\n", argv[0]);
+    fprintf(stderr, "usage: %s <model> [-d|-p] [--por] [--por-method cample|ample|stubborn] [--cycle-proviso] [--max-states N] [--threads T] [--expect N]\n", argv[0]);
     return 1;
   }
   orc_options o;
   memset(&o, 0, sizeof o);
+  int threads = 1;
   for (int i = 2; i < argc; i++) {
     if (!strcmp(argv[i], "-d")) o.property = ORC_PROP_DEADLOCK;
     else if (!strcmp(argv[i], "-p")) o.property = ORC_PROP_SAFETY;
@@ -38,6 +41,8 @@ int main(int argc, char **argv) {
     }
     else if (!strcmp(argv[i], "--max-states") && i + 1 < argc) o.max_states = strtoull(argv[++i], NULL, 10);
     else if (!strcmp(argv[i], "--seed") && i + 1 < argc) o.hash_seed = strtoull(argv[++i], NULL, 10);
+    else if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--expect") && i + 1 < argc) o.expected_states = strtoull(argv[++i], NULL, 10);
     else {
       fprintf(stderr, "unknown option %s\n", argv[i]);
       return 1;
@@ -50,7 +55,7 @@ int main(int argc, char **argv) {
     return 1;
   }
   orc_result r;
-  rc = orc_explore(m, &o, &r);
+  rc = threads > 1 ? orc_explore_mt(m, &o, threads, &r) : orc_explore(m, &o, &r);
   if (rc) {
     fprintf(stderr, "explore failed: %d\n", rc);
     orc_free(m);

@@ -27,6 +27,7 @@ class _Opts(ctypes.Structure):
         ("max_states", ctypes.c_uint64),
         ("hash_seed", ctypes.c_uint64),
         ("por_method", ctypes.c_int),
+        ("expected_states", ctypes.c_uint64),
     ]
 
 

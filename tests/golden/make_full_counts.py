@@ -2,10 +2,12 @@
 full-size generated workload, with the SHA-256 of the model text so a change
 of the generator is caught before a count comparison is made.
 
-    python tests/golden/make_full_counts.py config2 [--flags -d ...]
+    python tests/golden/make_full_counts.py config2 [--flags -d ...] [--threads T --expect N]
 
 A full-size run of the oracle takes tens of minutes (config2: ~35 min and
-~10 GB of host memory on one core); the GPU tests compare against the JSON.
+~10 GB of host memory on one core; config3 unreduced, 945 M states: ~26 GB
+and about 20 min on 8 threads with --expect presizing the level-free visited
+set); the GPU tests compare against the JSON.
 """
 
 import argparse
@@ -30,6 +32,8 @@ def main() -> int:
     ap.add_argument("--flags", nargs="*", default=[])
     ap.add_argument("--max-states", type=int, default=0,
                     help="stop after the first level that reaches this many states")
+    ap.add_argument("--threads", type=int, default=1, help="orc_explore_mt threads (plain method only)")
+    ap.add_argument("--expect", type=int, default=0, help="presize the visited set for this many states")
     args = ap.parse_args()
     net = WORKLOADS[args.workload]()
     text = net.to_gpf()
@@ -40,6 +44,10 @@ def main() -> int:
         with open(base + ".gpf", "w") as f:
             f.write(text)
         extra = ["--max-states", str(args.max_states)] if args.max_states else []
+        if args.threads > 1:
+            extra += ["--threads", str(args.threads)]
+        if args.expect:
+            extra += ["--expect", str(args.expect)]
         out = subprocess.run([cli, base] + args.flags + extra, check=True, capture_output=True,
                              text=True).stdout
     kv = dict(tok.split("=") for tok in out.split())
@@ -53,7 +61,7 @@ def main() -> int:
         "states": int(kv["states"]), "transitions": int(kv["transitions"]),
         "levels": int(kv["levels"]), "violation": bool(int(kv["violation"])),
         "truncated": bool(int(kv["truncated"])), "max_states": args.max_states,
-        "oracle_seconds": float(kv["seconds"])}
+        "oracle_seconds": float(kv["seconds"]), "oracle_threads": args.threads}
     with open(path, "w") as f:
         json.dump(data, f, indent=1)
     print(path, data)

@@ -78,3 +78,40 @@ def test_serve_restarts_on_tiny_estimates(gpu, jobs):
     res = serve(gpu, jobs[:12], estimate=1, slots=8, table_bytes=1 << 28, page_bytes=1 << 16)
     check(jobs[:12], res)
     assert any(r["restarts"] > 0 for r in res.values())
+
+
+def test_serve_chunk_beyond_candidate_buffer(gpu, jobs, tmp_path):
+    """A job one of whose 64-state chunks emits more successors than the
+    whole candidate buffer fails alone with GX_E_CAPACITY; the server keeps
+    running, and a job that fits completes with its exact counts."""
+    say("serve chunk beyond the candidate buffer")
+    small = str(tmp_path / "cycle5")
+    gen.independent_cycles([5]).write(small)  # one successor per state
+    big_path = str(tmp_path / "ring6")
+    gen.config1(1).write(big_path)  # 13,440 states: wide levels of out-degree 2
+    srv = gpu.Server(slots=4, table_bytes=1 << 26, page_bytes=1 << 16, cand_capacity=64)
+    j_big = srv.submit(gpu.Model.load(big_path), 0, 20000)
+    j_small = srv.submit(gpu.Model.load(small), 0, 5)
+    done = {}
+    for _ in range(1000):
+        srv.step()
+        for r in srv.poll():
+            done[r["job_id"]] = r
+        if len(done) == 2:
+            break
+    assert set(done) == {j_big, j_small}
+    assert done[j_big]["status"] == -10 and done[j_big]["restarts"] == 0
+    s = done[j_small]
+    alone = gpu.explore(gpu.Model.load(small), q_ints=1 << 16)
+    assert s["status"] == 0 and (s["states"], s["transitions"]) == (5, 5)  # 1 cycle of 5
+    assert s["levels"] == alone["levels"]
+    # the server still admits and completes work afterwards
+    j3 = srv.submit(gpu.Model.load(small), 0, 5)
+    for _ in range(1000):
+        srv.step()
+        got = {r["job_id"]: r for r in srv.poll()}
+        if j3 in got:
+            assert got[j3]["status"] == 0 and got[j3]["states"] == 5
+            break
+    else:
+        raise AssertionError("server stuck after a candidate overflow")
