@@ -19,8 +19,9 @@ CLI_PATH = os.path.join(HERE, "bin", "GPUexplore")
 GX_OK = 0
 ERRORS = {
     -1: "NOFILE", -2: "FORMAT", -3: "RANGE", -4: "NOMEM", -5: "HIP", -6: "NODEVICE",
-    -7: "ARG", -8: "LIMIT", -9: "STATE", -10: "CAPACITY",
+    -7: "ARG", -8: "LIMIT", -9: "STATE", -10: "CAPACITY", -11: "SCRATCH",
 }
+NOALLOC_MAX_KEYS = 262144  # GX_NOALLOC_MAX_KEYS
 PROP_NONE, PROP_DEADLOCK, PROP_SAFETY, PROP_LIVENESS = 0, 1, 2, 3
 POR_CAMPLE, POR_AMPLE, POR_STUBBORN = 0, 1, 2  # gx_options.por_method
 
@@ -181,6 +182,9 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_uint64)]),
     "gx_table_insert_device": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_float)]),
+    "gx_table_insert_host_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                                    ctypes.POINTER(ctypes.c_uint8),
+                                                    ctypes.POINTER(ctypes.c_uint64)]),
     "gx_table_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "gx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "gx_device_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -389,6 +393,19 @@ class Table:
         flags = np.zeros(max(n, 1), dtype=np.uint8)
         ins = ctypes.c_uint64(0)
         rc = lib().gx_table_insert_host(
+            self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ins))
+        return flags[:n], ins.value, rc
+
+    def insert_noalloc(self, keys: Sequence[int]):
+        """gx_table_insert_host_noalloc (at most NOALLOC_MAX_KEYS keys, staged
+        in the library's fixed scratch).  Returns (is_new, inserted, status)."""
+        import numpy as np
+        a = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+        n = a.size // self.key_words
+        flags = np.zeros(max(n, 1), dtype=np.uint8)
+        ins = ctypes.c_uint64(0)
+        rc = lib().gx_table_insert_host_noalloc(
             self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
             flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ins))
         return flags[:n], ins.value, rc

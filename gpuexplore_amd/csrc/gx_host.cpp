@@ -9,6 +9,7 @@
 #include <cmath>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <unordered_set>
@@ -155,6 +156,7 @@ extern "C" const char *gx_strerror(int s) {
     case GX_E_LIMIT: return "model exceeds a supported limit";
     case GX_E_STATE: return "reached a local state without transition offsets";
     case GX_E_CAPACITY: return "frontier or successor buffer exhausted";
+    case GX_E_SCRATCH: return "input larger than the no-allocation scratch buffer";
     default: return "unknown error";
   }
 }
@@ -605,6 +607,7 @@ struct gx_table {
   uint32_t W = 1, BW = 16;
   uint32_t cus = 1;       // compute units of the table's device
   int dev = 0;            // the device the table lives on
+  void *scratch = nullptr;  // that device's no-allocation scratch (device address)
   uint64_t seed = 0;
   hipStream_t st = nullptr;
   ~gx_table() {
@@ -614,6 +617,39 @@ struct gx_table {
     }
   }
 };
+
+namespace {
+
+// Scratch of gx_table_insert_host_noalloc: the control block, the staged keys
+// and their is-new bytes, in one file-scope device array.  Every device that
+// loads the library holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device
+// address comes from hipGetSymbolAddress, once per device (scratch_of).
+constexpr size_t kScratchCtrl = (sizeof(Ctrl) + 255) / 256 * 256;
+constexpr size_t kScratchKeys = static_cast<size_t>(GX_NOALLOC_MAX_KEYS) * kMaxKeyWords * 8;
+constexpr size_t kScratchBytes = kScratchCtrl + kScratchKeys + GX_NOALLOC_MAX_KEYS;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];  // serialises the calls that use a device's scratch
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_init_mu;
+
+// device address of `scratch` on device `dev` (current), resolved once
+int scratch_of(int dev, void **out) {
+  if (dev < 0 || dev >= kMaxDevices) return GX_E_LIMIT;
+  std::lock_guard<std::mutex> g(g_scratch_init_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    GX_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return GX_E_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GX_OK;
+}
+
+}  // namespace
 
 extern "C" int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t bucket_bytes,
                                uint64_t hash_seed, gx_table **out) {
@@ -634,6 +670,9 @@ extern "C" int gx_table_create(uint64_t q_ints, uint32_t key_words, uint32_t buc
   hipDeviceProp_t prop;
   GX_TRY(hipGetDeviceProperties(&prop, t->dev));
   t->cus = static_cast<uint32_t>(prop.multiProcessorCount);
+  // set-up of the no-allocation entry point on this device
+  const int src = scratch_of(t->dev, &t->scratch);
+  if (src != GX_OK) return src;
   if (hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) != hipSuccess) return GX_E_HIP;
   if (t->words.alloc(t->nbuckets * bucket_bytes) != hipSuccess) return GX_E_NOMEM;
   if (t->ctrl.alloc(sizeof(Ctrl)) != hipSuccess) return GX_E_NOMEM;
@@ -710,6 +749,27 @@ extern "C" int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t 
   int rc = table_insert(t, t->ctrl.as<Ctrl>(), k.as<uint64_t>(), n, s.as<uint8_t>(), false, inserted, nullptr);
   if (is_new && n) {
     GX_TRY(hipMemcpyAsync(is_new, s.p, n, hipMemcpyDeviceToHost, t->st));
+    GX_TRY(hipStreamSynchronize(t->st));
+  }
+  return rc;
+}
+
+extern "C" int gx_table_insert_host_noalloc(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
+                                            uint64_t *inserted) {
+  if (!t || (!keys && n)) return GX_E_ARG;
+  if (n > GX_NOALLOC_MAX_KEYS) return GX_E_SCRATCH;
+  int dev = 0;
+  GX_TRY(hipGetDevice(&dev));
+  if (dev != t->dev || !t->scratch) return GX_E_ARG;  // the table's scratch is on its own device
+  std::lock_guard<std::mutex> g(g_scratch_mu[dev]);
+  unsigned char *base = static_cast<unsigned char *>(t->scratch);
+  Ctrl *ctrl = reinterpret_cast<Ctrl *>(base);
+  uint64_t *dkeys = reinterpret_cast<uint64_t *>(base + kScratchCtrl);
+  uint8_t *dstat = base + kScratchCtrl + kScratchKeys;
+  if (n) GX_TRY(hipMemcpyAsync(dkeys, keys, n * t->W * 8, hipMemcpyHostToDevice, t->st));
+  const int rc = table_insert(t, ctrl, dkeys, n, dstat, true, inserted, nullptr);
+  if (is_new && n) {
+    GX_TRY(hipMemcpyAsync(is_new, dstat, n, hipMemcpyDeviceToHost, t->st));
     GX_TRY(hipStreamSynchronize(t->st));
   }
   return rc;

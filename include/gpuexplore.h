@@ -35,6 +35,7 @@ extern "C" {
 #define GX_E_LIMIT (-8)     /* model exceeds a supported limit */
 #define GX_E_STATE (-9)     /* a reached local state has no offsets in the gpf */
 #define GX_E_CAPACITY (-10) /* frontier or candidate buffer exhausted */
+#define GX_E_SCRATCH (-11)  /* input larger than the no-allocation scratch holds */
 
 /* properties (cu:27) */
 #define GX_PROP_NONE 0
@@ -270,6 +271,18 @@ int gx_table_insert_host(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t 
  * non-NULL. */
 int gx_table_insert_device(gx_table *t, const uint64_t *dev_keys, uint64_t n,
                            uint64_t *inserted, float *kernel_ms);
+
+/* gx_table_insert_host without any device allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate (serving loops,
+ * harnesses that forbid hipMalloc).  Keys, is-new bytes and the control block
+ * are staged in a fixed scratch array of the library, one per device (about
+ * 10.3 MiB, held until the process ends), so at most GX_NOALLOC_MAX_KEYS keys
+ * per call (GX_E_SCRATCH beyond).  The table must live on the current device.
+ * Calls on one device share its scratch: the library serialises them, and
+ * each call has finished with the scratch when it returns. */
+#define GX_NOALLOC_MAX_KEYS 262144u
+int gx_table_insert_host_noalloc(gx_table *t, const uint64_t *keys, uint64_t n, uint8_t *is_new,
+                                 uint64_t *inserted);
 
 /* Number of occupied slots (cu:687-704). */
 int gx_table_count(gx_table *t, uint64_t *count);
