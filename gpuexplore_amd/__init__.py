@@ -115,8 +115,23 @@ class ServerOptions(ctypes.Structure):
         ("cand_capacity", ctypes.c_uint64),
         ("hash_seed", ctypes.c_uint64),
         ("model_words", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("por_jobs", ctypes.c_uint32),
     ]
+
+
+class JobOptions(ctypes.Structure):
+    """gx_job_options: property, POR, cycle proviso and POR method of one job."""
+    _fields_ = [
+        ("property", ctypes.c_int32),
+        ("por", ctypes.c_int32),
+        ("cycle_proviso", ctypes.c_int32),
+        ("por_method", ctypes.c_uint32),
+    ]
+
+
+def job_options(prop: int = 0, por: bool = False, cycle_proviso: bool = False,
+                por_method: int = 0) -> JobOptions:
+    return JobOptions(prop, int(bool(por)), int(bool(cycle_proviso)), por_method)
 
 
 class ServerStatus(ctypes.Structure):
@@ -168,6 +183,8 @@ SIGNATURES = {
     "gx_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),
     "gx_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
     "gx_server_submit": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "gx_server_submit_job": (ctypes.c_int, [_P, _P, ctypes.POINTER(JobOptions), ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "gx_server_step": (ctypes.c_int, [_P, ctypes.POINTER(ServerStatus)]),
     "gx_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(JobResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "gx_server_free": (None, [_P]),
@@ -175,6 +192,10 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_uint64), ctypes.c_int,
                                         ctypes.POINTER(BatchOptions), ctypes.POINTER(Result),
                                         ctypes.POINTER(BatchStats)]),
+    "gx_explore_batch_jobs": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(JobOptions),
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.c_int,
+                                             ctypes.POINTER(BatchOptions), ctypes.POINTER(Result),
+                                             ctypes.POINTER(BatchStats)]),
     "gx_table_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                        ctypes.POINTER(_P)]),
     "gx_table_free": (None, [_P]),
@@ -192,6 +213,9 @@ SIGNATURES = {
     "gx_copy_to_device": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "gx_copy_to_host": (ctypes.c_int, [_P, _P, ctypes.c_uint64]),
     "gx_bench_copy": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "gx_bench_read": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "gx_bench_random": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -311,21 +335,26 @@ class Explorer:
             self.h = None
 
 
-def explore_batch(models, properties=None, weights=None, **kw):
-    """gx_explore_batch.  Returns (list of per-instance result dicts, stats
-    dict, status)."""
+def explore_batch(models, properties=None, weights=None, jobs=None, **kw):
+    """gx_explore_batch (or gx_explore_batch_jobs when ``jobs`` -- a list of
+    JobOptions, see job_options() -- is given).  Returns (list of
+    per-instance result dicts, stats dict, status)."""
     n = len(models)
     o = BatchOptions()
     lib().gx_batch_options_default(ctypes.byref(o))
     _set_fields(o, kw)
     hs = (ctypes.c_void_p * n)(*[m.h.value for m in models])
-    props = (ctypes.c_int32 * n)(*(properties or [0] * n))
     w = (ctypes.c_uint64 * n)(*weights) if weights is not None else None
+    wp = ctypes.cast(w, ctypes.POINTER(ctypes.c_uint64)) if w is not None else None
     res = (Result * n)()
     st = BatchStats()
-    rc = lib().gx_explore_batch(ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p)), props,
-                                ctypes.cast(w, ctypes.POINTER(ctypes.c_uint64)) if w is not None else None,
-                                n, ctypes.byref(o), res, ctypes.byref(st))
+    mp = ctypes.cast(hs, ctypes.POINTER(ctypes.c_void_p))
+    if jobs is not None:
+        js = (JobOptions * n)(*jobs)
+        rc = lib().gx_explore_batch_jobs(mp, js, wp, n, ctypes.byref(o), res, ctypes.byref(st))
+    else:
+        props = (ctypes.c_int32 * n)(*(properties or [0] * n))
+        rc = lib().gx_explore_batch(mp, props, wp, n, ctypes.byref(o), res, ctypes.byref(st))
     if rc not in (GX_OK, -10):
         raise GxError(rc, "explore_batch")
     return [r.as_dict() for r in res], {k: getattr(st, k) for k, _ in st._fields_}, rc
@@ -343,9 +372,13 @@ class Server:
         self.h = h
         self._models = {}  # job id -> Model (kept alive until the job is polled)
 
-    def submit(self, model: Model, prop: int = 0, est_states: int = 1) -> int:
+    def submit(self, model: Model, prop: int = 0, est_states: int = 1, por: bool = False,
+               cycle_proviso: bool = False, por_method: int = 0) -> int:
+        """gx_server_submit_job with the job's property and POR options."""
         jid = ctypes.c_uint64(0)
-        _check(lib().gx_server_submit(self.h, model.h, prop, est_states, ctypes.byref(jid)), "submit")
+        j = job_options(prop, por, cycle_proviso, por_method)
+        _check(lib().gx_server_submit_job(self.h, model.h, ctypes.byref(j), est_states, ctypes.byref(jid)),
+               "submit")
         self._models[jid.value] = model
         return jid.value
 

@@ -8,16 +8,24 @@
 
 namespace gx {
 
-// One instance (model exploration) of a batch or a server slot.
+// One instance (model exploration) of a batch or a server slot.  A plain
+// instance is expanded in chunks of 64 states (one state per lane); a POR
+// instance (mode kModePor or kModePorProviso) in chunks of G = floor(64/N)
+// states, one process-group layout pass (gx_group.h) per chunk.  The two
+// kinds have their own chunk prefixes, so each expand kernel walks only its
+// own chunks.
 struct BatchInst {
   ModelHdr hdr;
   uint32_t model_off;   // into the concatenated model buffer (words)
   int32_t property;
+  uint32_t mode;        // ExpandMode: plain, POR, POR with the cycle proviso
+  uint32_t por_method;  // GX_POR_* under POR
   uint64_t tb, nb;      // visited-set partition: first bucket, buckets
   uint64_t fo, fcap;    // frontier region: first entry, capacity
   uint64_t front_count[2];
   uint64_t inserts, transitions, expanded;
-  uint64_t chunk_prefix;  // first chunk of this instance in the round
+  uint64_t chunk_prefix;      // first plain chunk of this instance in the round
+  uint64_t chunk_prefix_por;  // first POR chunk of this instance in the round
   uint32_t cur, level;
   uint32_t violation, table_full, overflow, bad_state, spin_timeout, stop;
   uint32_t cand_overflow;  // a 64-state chunk of this instance emits more successors than the
@@ -29,7 +37,8 @@ struct BatchInst {
 struct BatchCtrl {
   uint64_t cand_count;
   uint64_t lookups;
-  uint64_t total_chunks;
+  uint64_t total_chunks;      // plain chunks of the round
+  uint64_t total_chunks_por;  // POR chunks of the round
   uint64_t deferred_count[2];
   uint32_t deferred_cur;
   uint32_t use_deferred;
@@ -52,12 +61,25 @@ struct BatchArgs {
   uint64_t *deferred[2];  // chunk descriptors: instance << 40 | first state
   uint64_t *table;
   uint64_t seed;
+  uint32_t bucket_words;  // 64-bit words per bucket
   uint32_t max_procs;
+  uint32_t rows16;   // every model has fewer than 0xFFFF rows: 16-bit row indices in LDS
+  uint32_t por;      // POR instances may run: the POR expand kernel is part of the round
+  uint32_t stamps;   // instances with the cycle proviso may run: slots carry level stamps
+  uint32_t por_blocks, por_threads;  // launch of the POR expand kernel
 };
 
-// expand + FIND-OR-PUT + advance of every unfinished instance (one BFS level)
+// Mode of an instance from its job options (gx_job_options).
+inline uint32_t batch_mode(int por, int cycle_proviso) {
+  return por ? (cycle_proviso ? kModePorProviso : kModePor) : kModePlain;
+}
+
+// expand (plain, and POR when a.por) + FIND-OR-PUT + advance of every
+// unfinished instance (one BFS level)
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
                           uint32_t ins_blocks, hipStream_t s);
+// LDS bytes of the plain batch expand kernel
+size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16);
 // the advance step alone (with prefix_only set: chunk prefix after admissions)
 hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s);
 // the initial state into a zeroed partition (first bucket tb, nb buckets)

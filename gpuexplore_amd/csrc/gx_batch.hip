@@ -22,17 +22,22 @@
 #include "../../include/gpuexplore.h"
 #include "gx_kernels.h"
 #include "gx_model.h"
+#include "gx_group.h"
 #include "gx_tps.h"
 #include "gx_batch.h"
 #include "gx_devbuf.h"
 
 namespace gx {
 
+// last instance whose chunk prefix (plain or POR) is <= c: instances without
+// chunks of that kind share the prefix of the next one, so the search lands on
+// the instance that owns chunk c
+template <bool POR>
 __device__ __forceinline__ uint32_t find_instance(const BatchInst *I, uint32_t n, uint64_t c) {
-  uint32_t lo = 0, hi = n;  // last instance with chunk_prefix <= c
+  uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (I[mid].chunk_prefix <= c)
+    if ((POR ? I[mid].chunk_prefix_por : I[mid].chunk_prefix) <= c)
       lo = mid;
     else
       hi = mid;
@@ -40,12 +45,17 @@ __device__ __forceinline__ uint32_t find_instance(const BatchInst *I, uint32_t n
   return lo;
 }
 
-template <int W>
+size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
+  return static_cast<size_t>(threads / 64) * max_procs * 64 * (rows16 ? 2 : 4);
+}
+
+// Plain instances: 64-state chunks, one state per lane.
+template <int W, typename RowT>
 __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
   extern __shared__ uint32_t smem[];
   BatchCtrl *C = a.ctrl;
   if (C->active == 0) return;
-  uint32_t *rows = smem + static_cast<uint32_t>(threadIdx.x >> 6) * (a.max_procs * 64u);
+  RowT *rows = reinterpret_cast<RowT *>(smem) + static_cast<uint32_t>(threadIdx.x >> 6) * (a.max_procs * 64u);
   const uint32_t n = C->n;
   const uint32_t use_def = C->use_deferred;
   const uint32_t dc = C->deferred_cur;
@@ -64,8 +74,9 @@ __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
       const uint64_t d = def_in[c];
       i = static_cast<uint32_t>(d >> 40);
       start = d & ((1ull << 40) - 1);
+      if (a.inst[i].mode != kModePlain) continue;  // the POR kernel's
     } else {
-      i = find_instance(a.inst, n, c);
+      i = find_instance<false>(a.inst, n, c);
       start = (c - a.inst[i].chunk_prefix) * 64;
     }
     BatchInst *I = &a.inst[i];
@@ -120,13 +131,138 @@ __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
   ch.release_all<W>();
 }
 
+// The slot format of an instance: with the cycle proviso a 32-bit level
+// stamp (the level its states are found at) sits after the state's K bits
+// and is left out of comparisons; otherwise every bit is compared (a stamp
+// position past the key's words places nothing).
+template <int W>
+__device__ __forceinline__ void batch_fmt(SlotFmt<W> &f, const BatchInst *I, uint32_t stamp) {
+  const uint32_t K = I->mode == kModePorProviso ? I->hdr.state_bits : 63u * W;
+  make_fmt<W, true>(f, K, stamp);
+}
+
+// POR instances: 64-state chunks in the process-group layout of the
+// single-model POR kernel (same cluster, ample or stubborn selection, cycle
+// proviso against the instance's own partition), G = floor(64/N) states per
+// pass.  A chunk is walked twice: the first walk only counts its successors,
+// so the chunk reserves its output once (or is deferred whole); the second
+// emits them.  Both walks select the same sets: the visited set does not
+// change while successors are generated.
+template <int W>
+__global__ __launch_bounds__(256) void k_batch_expand_por(BatchArgs a) {
+  extern __shared__ uint32_t smem[];
+  BatchCtrl *C = a.ctrl;
+  if (C->active == 0) return;
+  const int wib = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  GroupScratch X;
+  X.xch = reinterpret_cast<uint2 *>(smem) + wib * 64;
+  X.xc = smem + wpb * 128 + wib * 64;
+  X.xf = smem + wpb * 192 + wib * 64;
+  const uint32_t n = C->n;
+  const uint32_t use_def = C->use_deferred;
+  const uint32_t dc = C->deferred_cur;
+  const uint64_t nchunks = use_def ? C->deferred_count[dc] : C->total_chunks_por;
+  const uint64_t *def_in = a.deferred[dc];
+  uint64_t *def_out = a.deferred[dc ^ 1u];
+  uint64_t *def_out_count = &C->deferred_count[dc ^ 1u];
+  const int lane = lane_id();
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
+  for (uint64_t c = gw; c < nchunks; c += nw) {
+    uint32_t i;
+    uint64_t start;
+    if (use_def) {
+      const uint64_t d = def_in[c];
+      i = static_cast<uint32_t>(d >> 40);
+      start = d & ((1ull << 40) - 1);
+      if (a.inst[i].mode == kModePlain) continue;  // the plain kernel's
+    } else {
+      i = find_instance<true>(a.inst, n, c);
+      start = (c - a.inst[i].chunk_prefix_por) * 64;
+    }
+    BatchInst *I = &a.inst[i];
+    const ModelHdr H = I->hdr;
+    const uint32_t *M = a.models + I->model_off;
+    const GroupLane L = group_lane(M, H);
+    const uint32_t cur = I->cur;
+    const uint64_t f = I->front_count[cur];
+    const uint64_t end = start + 64 < f ? start + 64 : f;
+    const Table T{a.table + I->tb * static_cast<uint64_t>(a.bucket_words), I->nb, a.seed};
+    SlotFmt<W> fmt;
+    batch_fmt<W>(fmt, I, 0u);
+    const bool proviso = I->mode == kModePorProviso;
+    const int method = static_cast<int>(I->por_method);
+    GroupAcc acc;
+    bool counting = true;
+    uint64_t total = 0, pos = 0;
+    auto reserve = [&](uint64_t t, uint64_t *ob) {
+      if (counting) {
+        total += t;
+        return 0;
+      }
+      *ob = pos;
+      pos += t;
+      return 1;
+    };
+    auto emit = [&](uint64_t q, const uint64_t (&sv)[W]) {
+#pragma unroll
+      for (int j = 0; j < W; j++) a.cand[q * W + j] = sv[j];
+      a.cand_inst[q] = i;
+    };
+    auto walk = [&]() {
+      for (uint64_t b = start; b < end; b += static_cast<uint64_t>(L.G)) {
+        const uint64_t si = b + static_cast<uint64_t>(L.g);
+        const bool has = L.in_group && si < end;
+        uint64_t key[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) key[j] = has ? a.front[cur][(I->fo + si) * W + j] : kFlag;
+        if (proviso)
+          group_expand<W, kModePorProviso, W>(M, H, L, X, key, has, fmt, T, static_cast<int>(a.bucket_words),
+                                              I->level, I->property, method, reserve, emit, acc);
+        else
+          group_expand<W, kModePor, W>(M, H, L, X, key, has, fmt, T, static_cast<int>(a.bucket_words), I->level,
+                                       I->property, method, reserve, emit, acc);
+      }
+    };
+    walk();  // count
+    if (ballot(acc.bad) && lane == 0) I->bad_state = 1;
+    uint64_t obase = 0;
+    const int ok = ch.reserve<W>(total, (nchunks - c + nw - 1) / nw, &obase);
+    if (ok < 0) {
+      if (lane == 0) {
+        I->cand_overflow = 1;
+        C->cand_overflow = 1;
+      }
+      continue;
+    }
+    if (ok == 0) {
+      if (lane == 0) {
+        const uint64_t at = atomicAdd(reinterpret_cast<unsigned long long *>(def_out_count), 1ull);
+        def_out[at] = (static_cast<uint64_t>(i) << 40) | start;
+      }
+      continue;
+    }
+    counting = false;
+    pos = obase;
+    walk();  // emit
+    if (ballot(acc.viol) && lane == 0) I->violation = 1;
+    if (lane == 0 && (acc.trans || acc.states)) {
+      atomicAdd(reinterpret_cast<unsigned long long *>(&I->transitions), static_cast<unsigned long long>(acc.trans));
+      atomicAdd(reinterpret_cast<unsigned long long *>(&I->expanded), static_cast<unsigned long long>(acc.states));
+    }
+  }
+  ch.release_all<W>();
+}
+
 // New keys are staged per wave in LDS together with their instance and
 // appended to that instance's next frontier in blocks, one atomicAdd per
 // block: candidates of one instance come in long runs, and a per-key atomic
 // on one instance's counter would serialise the device.
 constexpr int kBatchStage = 128;
 
-template <int W, int BW>
+template <int W, int BW, bool STAMPS>
 __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
   extern __shared__ uint64_t bstage_all[];
   BatchCtrl *C = a.ctrl;
@@ -135,7 +271,7 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
   constexpr int LPB = BW / 2;
   constexpr int CPW = 64 / LPB;
   SlotFmt<W> fmt;
-  make_fmt<W, false>(fmt, 0, 0);
+  make_fmt<W, false>(fmt, 0, 0);  // every bit compared, no stamp (instances without the proviso)
   const int lane = lane_id();
   const int sub = lane % LPB;
   const int cg = lane / LPB;
@@ -185,7 +321,17 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
     const uint32_t i = active ? a.cand_inst[ci] : 0u;
     const BatchInst *I = &a.inst[i];
     Table T{a.table + (active ? I->tb : 0) * BW, active ? I->nb : 1, a.seed};
-    const int st = find_or_put_group<W, BW>(T, key, fmt, key_hash<W>(key, a.seed), sub, gbase, active);
+    int st;
+    if (STAMPS) {
+      // new states are stamped with the level they are found at
+      SlotFmt<W> f;
+      batch_fmt<W>(f, I, I->level + 1u);
+      uint64_t v[W];
+      slot_value<W, W>(key, f, v);
+      st = find_or_put_group<W, BW>(T, v, f, key_hash<W>(key, a.seed), sub, gbase, active);
+    } else {
+      st = find_or_put_group<W, BW>(T, key, fmt, key_hash<W>(key, a.seed), sub, gbase, active);
+    }
     acc_look += __popcll(ballot(active && sub == 0));
     if (active && sub == 0) {
       if (st == kFull) a.inst[i].table_full = 1;
@@ -215,8 +361,10 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
 }
 
 // One workgroup of 1024 threads; instances are handled in tiles of 1024.
+// Per instance: the level step and termination; then inclusive scans of the
+// next round's plain and POR chunk counts.
 __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
-  __shared__ uint64_t scan[1024];
+  __shared__ uint64_t scan[2][1024];
   __shared__ uint32_t nact;
   BatchCtrl *C = a.ctrl;
   if (C->active == 0 && C->prefix_only == 0) return;
@@ -244,15 +392,13 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   }
   __syncthreads();
   const uint32_t n = C->n;
-  uint64_t carry = 0;
+  uint64_t carry[2] = {0, 0};
   for (uint32_t t0 = 0; t0 < n; t0 += 1024) {
     const uint32_t i = t0 + tid;
-    uint64_t chunks = 0;
+    uint64_t chunks[2] = {0, 0};  // plain, POR
     if (i < n) {
       BatchInst *I = &a.inst[i];
-      if (!I->stop && prefix_only) {
-        chunks = (I->front_count[I->cur] + 63) / 64;
-      } else if (!I->stop) {
+      if (!I->stop && !prefix_only) {
         const uint32_t cur = I->cur, nxt = cur ^ 1u;
         const uint64_t fresh = I->front_count[nxt] < I->fcap ? I->front_count[nxt] : I->fcap;
         I->inserts += fresh;
@@ -262,40 +408,65 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
         if (I->violation || I->table_full || I->overflow || I->bad_state || I->spin_timeout || I->cand_overflow)
           I->stop = 1;
         if (I->front_count[nxt] == 0) I->stop = 1;
-        if (!I->stop) chunks = (I->front_count[nxt] + 63) / 64;
       }
-      if (!I->stop) atomicAdd(&nact, 1u);
+      if (!I->stop) {
+        const uint64_t f = I->front_count[I->cur];
+        chunks[I->mode == kModePlain ? 0 : 1] = (f + 63) / 64;
+        atomicAdd(&nact, 1u);
+      }
     }
-    scan[tid] = chunks;
+    for (int k = 0; k < 2; k++) scan[k][tid] = chunks[k];
     __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive scan
-      const uint64_t v = tid >= d ? scan[tid - d] : 0;
+    for (uint32_t d = 1; d < 1024; d <<= 1) {  // inclusive scans
+      const uint64_t v0 = tid >= d ? scan[0][tid - d] : 0;
+      const uint64_t v1 = tid >= d ? scan[1][tid - d] : 0;
       __syncthreads();
-      scan[tid] += v;
+      scan[0][tid] += v0;
+      scan[1][tid] += v1;
       __syncthreads();
     }
-    if (i < n) a.inst[i].chunk_prefix = carry + scan[tid] - chunks;
-    carry += scan[1023];
+    if (i < n) {
+      a.inst[i].chunk_prefix = carry[0] + scan[0][tid] - chunks[0];
+      a.inst[i].chunk_prefix_por = carry[1] + scan[1][tid] - chunks[1];
+    }
+    carry[0] += scan[0][1023];
+    carry[1] += scan[1][1023];
     __syncthreads();
   }
   if (tid == 0) {
-    C->total_chunks = carry;
+    C->total_chunks = carry[0];
+    C->total_chunks_por = carry[1];
     C->active = nact;
     C->prefix_only = 0;
   }
 }
 
+template <int W, int BW>
+static void launch_insert(const BatchArgs &a, uint32_t blocks, hipStream_t s) {
+  const uint32_t it = kInsertThreads;
+  const size_t ishm = static_cast<size_t>(it / 64) * kBatchStage * W * 8;
+  if (a.stamps)
+    hipLaunchKernelGGL((k_batch_insert<W, BW, true>), dim3(blocks), dim3(it), ishm, s, a);
+  else
+    hipLaunchKernelGGL((k_batch_insert<W, BW, false>), dim3(blocks), dim3(it), ishm, s, a);
+}
+
 template <int W>
 hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, uint32_t ins_blocks,
                         hipStream_t s) {
-  const size_t shm = static_cast<size_t>(threads / 64) * a.max_procs * 64 * 4;
-  hipLaunchKernelGGL((k_batch_expand<W>), dim3(blocks), dim3(threads), shm, s, a);
-  const uint32_t it = kInsertThreads;
-  const size_t ishm = static_cast<size_t>(it / 64) * kBatchStage * W * 8;
+  const size_t shm = batch_expand_lds(threads, a.max_procs, a.rows16 != 0);
+  if (a.rows16)
+    hipLaunchKernelGGL((k_batch_expand<W, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
+  else
+    hipLaunchKernelGGL((k_batch_expand<W, uint32_t>), dim3(blocks), dim3(threads), shm, s, a);
+  if (a.por) {
+    const size_t pshm = static_cast<size_t>(a.por_threads / 64) * kGroupScratchBytesPerWave;
+    hipLaunchKernelGGL((k_batch_expand_por<W>), dim3(a.por_blocks), dim3(a.por_threads), pshm, s, a);
+  }
   switch (BW) {
-    case 8: hipLaunchKernelGGL((k_batch_insert<W, 8>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
-    case 16: hipLaunchKernelGGL((k_batch_insert<W, 16>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
-    case 32: hipLaunchKernelGGL((k_batch_insert<W, 32>), dim3(ins_blocks), dim3(it), ishm, s, a); break;
+    case 8: launch_insert<W, 8>(a, ins_blocks, s); break;
+    case 16: launch_insert<W, 16>(a, ins_blocks, s); break;
+    case 32: launch_insert<W, 32>(a, ins_blocks, s); break;
     default: return hipErrorInvalidValue;
   }
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
@@ -304,14 +475,14 @@ hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t th
 
 static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStream_t s) {
   // the initial states: one candidate per instance
-#define GX_BI(Wk)                                                                               \
-  case Wk:                                                                                      \
-    switch (BW) {                                                                               \
-      case 8: hipLaunchKernelGGL((k_batch_insert<Wk, 8>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break;   \
-      case 16: hipLaunchKernelGGL((k_batch_insert<Wk, 16>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break; \
-      case 32: hipLaunchKernelGGL((k_batch_insert<Wk, 32>), dim3(64), dim3(256), 4 * kBatchStage * Wk * 8, s, a); break; \
-      default: return hipErrorInvalidValue;                                                     \
-    }                                                                                           \
+#define GX_BI(Wk)                                           \
+  case Wk:                                                  \
+    switch (BW) {                                           \
+      case 8: launch_insert<Wk, 8>(a, 64, s); break;        \
+      case 16: launch_insert<Wk, 16>(a, 64, s); break;      \
+      case 32: launch_insert<Wk, 32>(a, 64, s); break;      \
+      default: return hipErrorInvalidValue;                 \
+    }                                                       \
     break;
   switch (W) {
     GX_BI(1)
@@ -408,28 +579,50 @@ extern "C" void gx_batch_options_default(gx_batch_options *o) {
 extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *properties,
                                 const uint64_t *weights, int n, const gx_batch_options *o,
                                 gx_result *results, gx_batch_stats *stats) {
-  if (!models || !o || !results || n <= 0 || n > (1 << 20)) return GX_E_ARG;
+  if (!models || n <= 0 || n > (1 << 20)) return GX_E_ARG;
+  std::vector<gx_job_options> jobs(n);
+  for (int i = 0; i < n; i++) {
+    if (models[i] && models[i]->por) return GX_E_ARG;  // plain method only (gx_explore_batch_jobs has POR)
+    jobs[i] = gx_job_options{properties ? properties[i] : GX_PROP_NONE, 0, 0, GX_POR_CAMPLE};
+  }
+  return gx_explore_batch_jobs(models, jobs.data(), weights, n, o, results, stats);
+}
+
+extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job_options *jobs,
+                                     const uint64_t *weights, int n, const gx_batch_options *o,
+                                     gx_result *results, gx_batch_stats *stats) {
+  if (!models || !jobs || !o || !results || n <= 0 || n > (1 << 20)) return GX_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return GX_E_NODEVICE;
   memset(results, 0, sizeof(gx_result) * n);
   if (stats) memset(stats, 0, sizeof *stats);
   // layout: common key width, concatenated models
   int W = 1;
-  uint32_t max_procs = 1;
+  uint32_t max_procs = 1, max_rows = 0;
+  bool any_por = false, any_stamp = false;
   std::vector<uint32_t> mbuf;
   std::vector<BatchInst> inst(n);
   for (int i = 0; i < n; i++) {
     const gx_model *m = models[i];
-    if (!m || m->por) return GX_E_ARG;
+    const gx_job_options &J = jobs[i];
+    if (!m) return GX_E_ARG;
+    if (J.property < GX_PROP_NONE || J.property > GX_PROP_LIVENESS || J.por_method > GX_POR_STUBBORN)
+      return GX_E_ARG;
     const ModelHdr &H = m->comp.hdr;
-    if (!H.lab64) return GX_E_LIMIT;
+    if (!J.por && !H.lab64) return GX_E_LIMIT;  // the state-per-lane kernel's label masks
     W = std::max<int>(W, static_cast<int>(H.key_words));
+    // a level stamp after the state bits needs K + 32 bits in the slot
+    if (J.por && J.cycle_proviso) W = std::max<int>(W, static_cast<int>((H.state_bits + 32 + 62) / 63));
     max_procs = std::max(max_procs, H.nprocs);
+    max_rows = std::max(max_rows, H.nrows);
+    any_por = any_por || J.por;
+    any_stamp = any_stamp || (J.por && J.cycle_proviso);
     memset(&inst[i], 0, sizeof(BatchInst));
     inst[i].hdr = H;
     inst[i].model_off = static_cast<uint32_t>(mbuf.size());
-    inst[i].property = properties ? properties[i] : GX_PROP_NONE;
-    if (inst[i].property < GX_PROP_NONE || inst[i].property > GX_PROP_LIVENESS) return GX_E_ARG;
+    inst[i].property = J.property;
+    inst[i].mode = batch_mode(J.por, J.cycle_proviso);
+    inst[i].por_method = J.por_method;
     inst[i].cur = 1;
     inst[i].level = 0xFFFFFFFFu;
     mbuf.insert(mbuf.end(), m->comp.words.begin(), m->comp.words.end());
@@ -443,7 +636,8 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   const uint64_t spb = static_cast<uint64_t>(BW / W);
   uint32_t threads = o->threads ? o->threads : 256;
   threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (threads + 63) / 64 * 64));
-  if (static_cast<size_t>(threads / 64) * max_procs * 256 > 64 * 1024) return GX_E_LIMIT;
+  const bool rows16 = max_rows < 0xFFFFu;
+  if (batch_expand_lds(threads, max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
   int dev = 0;
   BX_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -522,7 +716,13 @@ extern "C" int gx_explore_batch(const gx_model *const *models, const int32_t *pr
   a.deferred[1] = static_cast<uint64_t *>(d_def1.p);
   a.table = static_cast<uint64_t *>(d_table.p);
   a.seed = o->hash_seed;
+  a.bucket_words = static_cast<uint32_t>(BW);
   a.max_procs = max_procs;
+  a.rows16 = rows16 ? 1u : 0u;
+  a.por = any_por ? 1u : 0u;
+  a.stamps = any_stamp ? 1u : 0u;
+  a.por_threads = 256;
+  a.por_blocks = prop.multiProcessorCount * 4;
   BX_TRY(launch_insert_only(a, W, BW, st));
   BX_TRY(hipStreamSynchronize(st));
   const double t_init = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_init0).count();

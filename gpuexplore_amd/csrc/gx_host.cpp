@@ -816,6 +816,41 @@ extern "C" int gx_copy_to_host(void *dst, const void *src, uint64_t bytes) {
   return GX_OK;
 }
 
+namespace {
+
+// Average time of `reps` launches of `launch` on the null stream after one
+// warm-up, in milliseconds.
+template <typename Launch>
+int time_launches(int reps, Launch &&launch, double *ms_out) {
+  GX_TRY(launch());  // warm-up
+  hipEvent_t e0, e1;
+  GX_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return GX_E_HIP;
+  }
+  hipError_t err = hipEventRecord(e0, nullptr);
+  for (int i = 0; i < reps && err == hipSuccess; i++) err = launch();
+  if (err == hipSuccess) err = hipEventRecord(e1, nullptr);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  float ms = 0;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  GX_TRY(err);
+  *ms_out = static_cast<double>(ms) / reps;
+  return GX_OK;
+}
+
+uint32_t device_cus() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+  return static_cast<uint32_t>(prop.multiProcessorCount);
+}
+
+}  // namespace
+
 extern "C" int gx_bench_copy(uint64_t bytes, int reps, double *gbps) {
   if (!gbps || bytes < 16 || reps < 1) return GX_E_ARG;
   if (device_count() == 0) return GX_E_NODEVICE;
@@ -824,23 +859,65 @@ extern "C" int gx_bench_copy(uint64_t bytes, int reps, double *gbps) {
   GX_TRY(a.alloc(n * 16));
   GX_TRY(b.alloc(n * 16));
   GX_TRY(hipMemset(a.p, 1, n * 16));
-  int dev = 0;
-  GX_TRY(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  GX_TRY(hipGetDeviceProperties(&prop, dev));
-  const uint32_t blocks = prop.multiProcessorCount * 16;
-  hipEvent_t e0, e1;
-  GX_TRY(hipEventCreate(&e0));
-  GX_TRY(hipEventCreate(&e1));
-  GX_TRY(launch_copy(a.as<uint4>(), b.as<uint4>(), n, blocks, nullptr));  // warm-up
-  GX_TRY(hipEventRecord(e0, nullptr));
-  for (int i = 0; i < reps; i++) GX_TRY(launch_copy(a.as<uint4>(), b.as<uint4>(), n, blocks, nullptr));
-  GX_TRY(hipEventRecord(e1, nullptr));
-  GX_TRY(hipEventSynchronize(e1));
-  float ms = 0;
-  GX_TRY(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *gbps = 2.0 * static_cast<double>(n * 16) * reps / (ms / 1e3) / 1e9;
+  const uint32_t cus = device_cus();
+  // the best of a few launch shapes: workgroups per CU x non-temporal hints
+  double best = 0;
+  for (uint32_t per_cu : {8u, 16u, 32u, 64u}) {
+    for (bool nt : {true, false}) {
+      double ms = 0;
+      const int rc = time_launches(
+          reps, [&]() { return launch_copy(a.as<uint4>(), b.as<uint4>(), n, cus * per_cu, nt, nullptr); }, &ms);
+      if (rc != GX_OK) return rc;
+      best = std::max(best, 2.0 * static_cast<double>(n * 16) / (ms / 1e3) / 1e9);
+    }
+  }
+  *gbps = best;
+  return GX_OK;
+}
+
+extern "C" int gx_bench_read(uint64_t bytes, int reps, double *gbps) {
+  if (!gbps || bytes < 16 || reps < 1) return GX_E_ARG;
+  if (device_count() == 0) return GX_E_NODEVICE;
+  const uint64_t n = bytes / 16;
+  DevBuf a, sink;
+  GX_TRY(a.alloc(n * 16));
+  GX_TRY(sink.alloc(4));
+  GX_TRY(hipMemset(a.p, 1, n * 16));
+  const uint32_t cus = device_cus();
+  double best = 0;
+  for (uint32_t per_cu : {8u, 16u, 32u, 64u}) {
+    double ms = 0;
+    const int rc = time_launches(
+        reps, [&]() { return launch_read(a.as<uint4>(), n, sink.as<uint32_t>(), cus * per_cu, nullptr); }, &ms);
+    if (rc != GX_OK) return rc;
+    best = std::max(best, static_cast<double>(n * 16) / (ms / 1e3) / 1e9);
+  }
+  *gbps = best;
+  return GX_OK;
+}
+
+extern "C" int gx_bench_random(uint64_t table_bytes, uint64_t lookups, int reps, double *lookups_per_s) {
+  if (!lookups_per_s || table_bytes < 64 || lookups < 1 || reps < 1) return GX_E_ARG;
+  if (device_count() == 0) return GX_E_NODEVICE;
+  const uint64_t nb = table_bytes / 64;
+  DevBuf t, sink;
+  GX_TRY(t.alloc(nb * 64));
+  GX_TRY(sink.alloc(4));
+  GX_TRY(hipMemset(t.p, 1, nb * 64));
+  const uint32_t cus = device_cus();
+  double best_ms = 0;
+  for (uint32_t per_cu : {16u, 32u, 64u}) {
+    double ms = 0;
+    const int rc = time_launches(
+        reps,
+        [&]() {
+          return launch_random_read(t.as<uint64_t>(), nb, lookups, 0x5eedull, sink.as<uint32_t>(), cus * per_cu,
+                                    nullptr);
+        },
+        &ms);
+    if (rc != GX_OK) return rc;
+    if (best_ms == 0 || ms < best_ms) best_ms = ms;
+  }
+  *lookups_per_s = static_cast<double>(lookups) / (best_ms / 1e3);
   return GX_OK;
 }

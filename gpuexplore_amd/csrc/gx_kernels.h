@@ -172,8 +172,13 @@ hipError_t launch_fold(Ctrl *c, hipStream_t s);
 hipError_t launch_count(const Table &t, int slot_words, int bucket_words, uint64_t *dev_count,
                         uint32_t blocks, hipStream_t s);
 
-// Streaming device-to-device copy of n 16-byte elements.
-hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks, hipStream_t s);
+// Streaming device-to-device copy of n 16-byte elements (nt: non-temporal
+// hints), streaming read of n 16-byte elements, and n random 64-byte bucket
+// reads from a table of nbuckets 64-byte buckets (measurement kernels).
+hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks, bool nt, hipStream_t s);
+hipError_t launch_read(const uint4 *src, uint64_t n, uint32_t *sink, uint32_t blocks, hipStream_t s);
+hipError_t launch_random_read(const uint64_t *table, uint64_t nbuckets, uint64_t n, uint64_t seed, uint32_t *sink,
+                              uint32_t blocks, hipStream_t s);
 
 // LDS bytes the expand kernel needs besides the staged model.
 size_t expand_scratch_bytes(uint32_t threads);

@@ -11,11 +11,11 @@
 // enqueued back to back without host round trips.
 #include "../../include/gpuexplore.h"
 #include "gx_kernels.h"
+#include "gx_group.h"
 #include "gx_tps.h"
 
 namespace gx {
 
-constexpr uint32_t kNone = 0xFFFFFFFFu;
 // New keys staged per wave before a flush (insert).  Each flush is one
 // atomicAdd on the next frontier's shared counter: 256 keys halve those
 // same-address atomics against 128 (FIND-OR-PUT 95 vs 101 ms on a ring of
@@ -51,20 +51,11 @@ __device__ __forceinline__ void add_counter(Ctrl *C, uint32_t which, uint64_t v)
 }
 
 // ---------------------------------------------------------------------------
-// Successor generation.
-//
-// A wave of 64 lanes expands G = floor(64/N) states at once: lane
-// (g*N + p) handles process p of the g-th state (the reference uses groups
-// of N lanes inside a 32-lane warp, cu:64-82).  Synchronising labels are
-// walked in ascending order; the enabled set En(a) is a ballot over the
-// group; rules of the label are split over the group's lanes and each lane
-// enumerates the product of target lists of its enabled rules, lowest
-// process varying fastest (cu:951-1130).
-//
-// Every state is walked in a counting pass and again in an emitting pass:
-// the counts give each lane its output offset (wave prefix sum) and the
-// wave reserves its whole output with one atomic.  A wave whose output does
-// not fit in the candidate buffer defers its states to a retry list.
+// Successor generation in the process-group layout (gx_group.h): G =
+// floor(64/N) states per wave, one lane per process.  POR, the cycle proviso
+// and models whose labels do not fit the state-per-lane kernel run here.  A
+// wave whose output does not fit in the candidate buffer defers its states
+// to a retry list.
 template <int W, bool LDS, int MODE, int WT>
 __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   extern __shared__ uint32_t smem[];
@@ -77,9 +68,10 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   const uint32_t xoff = LDS ? ((H.total_words + 3u) & ~3u) : 0u;
   const int wib = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  uint2 *xch = reinterpret_cast<uint2 *>(smem + xoff) + wib * 64;
-  uint32_t *xc = smem + xoff + wpb * 128 + wib * 64;
-  uint32_t *xf = smem + xoff + wpb * 192 + wib * 64;
+  GroupScratch X;
+  X.xch = reinterpret_cast<uint2 *>(smem + xoff) + wib * 64;
+  X.xc = smem + xoff + wpb * 128 + wib * 64;
+  X.xf = smem + xoff + wpb * 192 + wib * 64;
   __syncthreads();
   if (C->stop) return;
 
@@ -92,338 +84,54 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
   uint64_t *retry_out_count = &C->retry_count[rc ^ 1u];
   const uint64_t *front = a.front[cur];
   const uint32_t level = C->level;
-
-  const int N = static_cast<int>(H.nprocs);
-  const int G = 64 / N;
+  const GroupLane L = group_lane(M, H);
+  const int G = L.G;
   const int lane = lane_id();
-  const int g = lane / N;
-  const int p = lane - g * N;
-  const bool in_group = g < G;
-  const int base = in_group ? g * N : 0;
-  const uint64_t nmask = (N >= 64) ? ~0ull : ((1ull << N) - 1ull);
-  const GroupRot rot = make_rot(N, base, p, in_group);
-
-  const uint32_t floc = in_group ? M[H.off_floc + p] : 0u;
-  const uint32_t rowbase = in_group ? M[H.off_rowbase + p] : 0u;
-  const uint32_t rowcnt = in_group ? M[H.off_rowcnt + p] : 0u;
-  const uint32_t fwidth = floc >> 16;
-  const uint32_t fmask = (fwidth >= 32) ? 0xFFFFFFFFu : ((1u << fwidth) - 1u);
-  const bool last_proc = in_group && p == N - 1;
-  const uint32_t wlast = M[H.off_floc + N - 1] >> 16;
-  const uint32_t fmask_last = (wlast >= 32) ? 0xFFFFFFFFu : ((1u << wlast) - 1u);
-  const bool safety = a.property == 2;
 
   SlotFmt<WT> fmt;
   make_fmt<WT, MODE == kModePorProviso>(fmt, H.state_bits, 0u);
 
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint64_t acc_trans = 0, acc_states = 0;
-  bool viol = false, bad = false;
+  GroupAcc acc;
   CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
 
   for (uint64_t b0 = gw * G; b0 < nwork; b0 += nw * G) {
-    const uint64_t si = b0 + g;
-    const bool has = in_group && si < nwork;
+    const uint64_t si = b0 + L.g;
+    const bool has = L.in_group && si < nwork;
     uint64_t fi = 0;
     if (has) fi = use_retry ? retry_in[si] : si;
     uint64_t key[W];
 #pragma unroll
     for (int j = 0; j < W; j++) key[j] = has ? front[fi * W + j] : kFlag;
-    uint32_t l0 = 0, l1 = 0, e0 = 0, e1 = 0, rf = 0;
-    if (has) {
-      const uint32_t sp = kget<W>(key, floc);
-      if (sp < rowcnt) {
-        const uint32_t row = rowbase + sp;
-        rf = M[H.off_rowflags + row];
-        if (rf & kRowBad) {
-          bad = true;
-        } else {
-          l0 = M[H.off_loc_start + row];
-          l1 = M[H.off_loc_start + row + 1];
-          e0 = M[H.off_syn_start + row];
-          e1 = M[H.off_syn_start + row + 1];
-        }
-      } else {
-        bad = true;
-      }
-    }
-
-    // Walk the synchronising labels of the G states in ascending order.
-    // on_label(cur, en, mine) runs on every lane of a group whose label is
-    // `cur`; on_rule(r, cur) on the lane owning an enabled rule r.
-    auto walk = [&](auto &&on_label, auto &&on_rule) {
-      uint32_t e = e0;
-      for (;;) {
-        const uint32_t mylab = (e < e1) ? M[H.off_syn_lab + e] : kNone;
-        const uint32_t curl = group_min(mylab, rot);
-        if (!ballot(in_group && curl != kNone)) break;
-        const bool mine = in_group && mylab == curl && curl != kNone;
-        const uint64_t en = (ballot(mine) >> base) & nmask;
-        xch[lane] = mine ? make_uint2(M[H.off_syn_tstart + e], M[H.off_syn_tcnt + e]) : make_uint2(0u, 0u);
-        wave_sync();
-        if (in_group && curl != kNone) {
-          on_label(curl, en, mine);
-          const uint32_t rs = M[H.off_rule_start + curl], re = M[H.off_rule_start + curl + 1];
-          for (uint32_t j = rs + static_cast<uint32_t>(p); j < re; j += static_cast<uint32_t>(N)) {
-            const uint32_t r = M[H.off_rules + j];
-            if ((static_cast<uint64_t>(r) & ~en) == 0) on_rule(r, curl);
-          }
-        }
-        wave_sync();
-        if (mine) e++;
-      }
-    };
-    auto rule_product = [&](uint32_t r) {
-      uint64_t prod = 1;
-      for (uint32_t rr = r; rr; rr &= rr - 1) prod *= xch[base + __ffs(rr) - 1].y;
-      return prod;
-    };
-    // does T_{N-1}(cur) contain local state 1 (successor violates safety)
-    auto last_hits_one = [&]() {
-      const uint2 t = xch[base + N - 1];
-      for (uint32_t i = 0; i < t.y; i++)
-        if ((M[H.off_syn_tgt + t.x + i] & fmask_last) == 1u) return true;
-      return false;
-    };
-
-    // ---------------- pass 1: enabledness, safety, (POR) clusters
-    bool any_enabled = false;
-    uint32_t cluster0 = in_group ? (1u << p) : 0u;
-    uint32_t stub = cluster0;  // stubborn-set relation d(p)
-    bool in_en = false;
-    uint32_t rnew = 0;  // enabled rules with a new successor (proviso)
-    bool local_new = false;
-    if (safety && last_proc) {
-      for (uint32_t t = l0; t < l1; t++)
-        if ((M[H.off_loc_tgt + t] & fmask) == 1u) viol = true;
-    }
-    walk(
-        [&](uint32_t curl, uint64_t en, bool mine) {
-          if (MODE != kModePlain && mine) {
-            const uint32_t rs = M[H.off_rule_start + curl], re = M[H.off_rule_start + curl + 1];
-            for (uint32_t j = rs; j < re; j++) {
-              const uint32_t r = M[H.off_rules + j];
-              if ((r >> p) & 1u) {
-                cluster0 |= r;
-                // the whole rule when every member has the label, else its
-                // lowest member without it (the moves that can enable it)
-                const uint32_t lack = r & ~static_cast<uint32_t>(en);
-                stub |= lack ? (lack & (~lack + 1u)) : r;
-                if ((static_cast<uint64_t>(r) & ~en) == 0 && rule_product(r) > 0) in_en = true;
-              }
-            }
-          }
-        },
-        [&](uint32_t r, uint32_t curl) {
-          any_enabled = true;
-          if (safety && ((r >> (N - 1)) & 1u) && last_hits_one()) viol = true;
-          if (MODE == kModePorProviso) {
-            // enumerate until a successor is new (not visited at level <= L)
-            const uint64_t prod64 = rule_product(r);
-            const uint32_t prod = prod64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(prod64);
-            bool found_new = false;
-            for (uint32_t k = 0; k < prod && !found_new; k++) {
-              uint64_t s[W];
-#pragma unroll
-              for (int j = 0; j < W; j++) s[j] = key[j];
-              uint32_t rem = k;
-              for (uint32_t rr = r; rr; rr &= rr - 1) {
-                const int q = __ffs(rr) - 1;
-                const uint2 t = xch[base + q];
-                const uint32_t d = rem % t.y;
-                rem /= t.y;
-                kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
-              }
-              uint64_t v[WT];
-              slot_value<W, WT>(s, fmt, v);
-              uint32_t stamp = 0;
-              const int f = find_thread<WT>(a.table, v, fmt, key_hash<W>(s, a.table.seed),
-                                            a.bucket_words, H.state_bits, &stamp);
-              if (!f || stamp > level) found_new = true;
-            }
-            if (found_new) rnew |= r;
-          }
-          (void)curl;
-        });
-    const bool local_entry = (rf & kRowHasLocal) != 0;
-    const bool dead = has && ((ballot(local_entry) >> base) & nmask) == 0 &&
-                      ((ballot(any_enabled) >> base) & nmask) == 0;
-    if (a.property == 1 && dead) viol = true;
-
-    // ---------------- POR: cluster selection (cu:1633-1672)
-    uint32_t csel = static_cast<uint32_t>(nmask);
-    if (MODE != kModePlain) {
-      bool prov;
-      if (MODE == kModePor) {
-        prov = (l1 > l0) || in_en;
-      } else {
-        for (uint32_t t = l0; t < l1 && !local_new; t++) {
-          uint64_t s[W];
-#pragma unroll
-          for (int j = 0; j < W; j++) s[j] = key[j];
-          kset<W>(s, floc, M[H.off_loc_tgt + t]);
-          uint64_t v[WT];
-          slot_value<W, WT>(s, fmt, v);
-          uint32_t stamp = 0;
-          const int f = find_thread<WT>(a.table, v, fmt, key_hash<W>(s, a.table.seed), a.bucket_words,
-                                        H.state_bits, &stamp);
-          if (!f || stamp > level) local_new = true;
-        }
-        // OR of the new-successor rules over the group
-        const uint32_t acc = group_or(rnew, rot);
-        prov = local_new || ((acc >> p) & 1u);
-      }
-      if (a.por_method == GX_POR_AMPLE) {
-        // the local moves of one process none of whose current labels has a
-        // rule containing it (cluster0 = {p}), with a (new) local
-        // successor; the fewest local successors win, the lowest on ties
-        const bool cand = in_group && cluster0 == (1u << p) &&
-                          (MODE == kModePorProviso ? local_new : (l1 > l0));
-        xf[lane] = cand ? (l1 - l0) : 0xFFFFFFFFu;
-        wave_sync();
-        if (in_group) {
-          uint32_t best = 0xFFFFFFFFu;
-          for (int q = 0; q < N; q++) {
-            const uint32_t nq = xf[base + q];
-            if (nq < best) {
-              best = nq;
-              csel = 1u << q;
-            }
-          }
-        }
-        wave_sync();
-      } else if (a.por_method == GX_POR_STUBBORN) {
-        // closure of {p} under d; valid when it meets a process with an
-        // enabled (under the proviso: new) successor; the smallest valid
-        // set below N processes wins, the lowest p on ties
-        const uint32_t target = static_cast<uint32_t>((ballot(in_group && prov) >> base) & nmask);
-        xc[lane] = stub;
-        wave_sync();
-        uint32_t K = in_group ? (1u << p) : 0u, done = 0;
-        if (in_group) {
-          while (K & ~done) {
-            const int i = __ffs(K & ~done) - 1;
-            done |= 1u << i;
-            K |= xc[base + i];
-          }
-        }
-        xf[lane] = (K & target) ? K : 0u;
-        wave_sync();
-        if (in_group) {
-          int best = N;
-          for (int q = 0; q < N; q++) {
-            const uint32_t fq = xf[base + q];
-            const int pc = __popc(fq);
-            if (fq && pc < best) {
-              best = pc;
-              csel = fq;
-            }
-          }
-        }
-        wave_sync();
-      } else {
-        const uint32_t c = prov ? cluster0 : (cluster0 & ~(1u << p));  // cu:1635-1637
-        xc[lane] = c;
-        wave_sync();
-        uint32_t K = c, done = 0;
-        bool ok = false;
-        if (in_group) {
-          while (K & ~done) {  // closure (cu:1641-1649)
-            const int i = __ffs(K & ~done) - 1;
-            done |= 1u << i;
-            const uint32_t ci = xc[base + i];
-            if ((ci >> i) & 1u) ok = true;
-            K |= ci;
-          }
-        }
-        xf[lane] = ok ? K : 0u;  // cu:1651-1655
-        wave_sync();
-        if (in_group) {
-          int best = N;
-          for (int q = 0; q < N; q++) {  // first smallest non-empty cluster (cu:1657-1666)
-            const uint32_t fq = xf[base + q];
-            const int pc = __popc(fq);
-            if (fq && pc < best) {
-              best = pc;
-              csel = fq;
-            }
-          }
-        }
-        wave_sync();
-      }
-    }
-
-    // ---------------- pass 2: count (ample) successors
-    uint64_t cnt = 0;
-    if (has && ((csel >> p) & 1u)) cnt += l1 - l0;
-    walk([&](uint32_t, uint64_t, bool) {},
-         [&](uint32_t r, uint32_t) {
-           if (has && (r & ~csel) == 0) cnt += rule_product(r);
-         });
-    uint64_t total = 0;
-    const uint64_t off = wave_excl_scan(cnt, &total);
-    uint64_t obase = 0;
     const uint64_t left = (nwork - b0 + nw * G - 1) / (nw * G);
-    const int ok = ch.reserve<W>(total, left, &obase);
+    const int ok = group_expand<W, MODE, WT>(
+        M, H, L, X, key, has, fmt, a.table, a.bucket_words, level, a.property, a.por_method,
+        [&](uint64_t total, uint64_t *obase) { return ch.reserve<W>(total, left, obase); },
+        [&](uint64_t pos, const uint64_t (&sv)[W]) {
+#pragma unroll
+          for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+        },
+        acc);
     if (ok <= 0) {
       // defer this wave's states to the retry list
       if (ok < 0 && lane == 0) C->overflow = 2;
-      const bool lead = has && p == 0;
+      const bool lead = has && L.p == 0;
       const uint64_t m = ballot(lead);
       uint64_t rb = 0;
       if (lane == 0) rb = atomicAdd(reinterpret_cast<unsigned long long *>(retry_out_count),
                                     static_cast<unsigned long long>(__popcll(m)));
       rb = __shfl(rb, 0, 64);
       if (lead) retry_out[rb + __popcll(m & ((1ull << lane) - 1ull))] = fi;
-      continue;
     }
-    acc_trans += total;
-    acc_states += __popcll(ballot(has && p == 0));
-
-    // ---------------- pass 3: emit
-    uint64_t pos = obase + off;
-    if (has && ((csel >> p) & 1u)) {
-      for (uint32_t t = l0; t < l1; t++) {
-        uint64_t s[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) s[j] = key[j];
-        kset<W>(s, floc, M[H.off_loc_tgt + t]);
-#pragma unroll
-        for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
-        pos++;
-      }
-    }
-    walk([&](uint32_t, uint64_t, bool) {},
-         [&](uint32_t r, uint32_t) {
-           if (!has || (r & ~csel) != 0) return;
-           // emitted products fit the candidate buffer, so < 2^32
-           const uint32_t prod = static_cast<uint32_t>(rule_product(r));
-           for (uint32_t k = 0; k < prod; k++) {
-             uint64_t s[W];
-#pragma unroll
-             for (int j = 0; j < W; j++) s[j] = key[j];
-             uint32_t rem = k;
-             for (uint32_t rr = r; rr; rr &= rr - 1) {
-               const int q = __ffs(rr) - 1;
-               const uint2 t = xch[base + q];
-               const uint32_t d = rem % t.y;
-               rem /= t.y;
-               kset<W>(s, M[H.off_floc + q], M[H.off_syn_tgt + t.x + d]);
-             }
-#pragma unroll
-             for (int j = 0; j < W; j++) a.cand[pos * W + j] = s[j];
-             pos++;
-           }
-         });
   }
   ch.release_all<W>();
   if (lane == 0) {
-    add_counter(C, kCntTransitions, acc_trans);
-    add_counter(C, kCntExpanded, acc_states);
+    add_counter(C, kCntTransitions, acc.trans);
+    add_counter(C, kCntExpanded, acc.states);
   }
-  if (ballot(viol) && lane == 0) C->violation = 1;
-  if (ballot(bad) && lane == 0) C->bad_state = 1;
+  if (ballot(acc.viol) && lane == 0) C->violation = 1;
+  if (ballot(acc.bad) && lane == 0) C->bad_state = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -947,7 +655,7 @@ __global__ void k_count(Table t, int WT, int BW, unsigned long long *out) {
 // ---------------------------------------------------------------------------
 size_t expand_scratch_bytes(uint32_t threads) {
   // per wave: 64 x uint2 exchange + 64 x uint32 clusters + 64 x uint32 finals
-  return static_cast<size_t>(threads / 64) * 64 * (8 + 4 + 4) + 16;
+  return static_cast<size_t>(threads / 64) * kGroupScratchBytesPerWave + 16;
 }
 
 template <int W, int WT>
@@ -1035,30 +743,93 @@ hipError_t launch_insert(const InsertArgs &a, int key_words, int slot_words, int
 #undef GX_INS
 }
 
-// Streaming copy used to measure the device's attainable bandwidth: four
-// independent 16-byte loads in flight per thread, non-temporal hints.
+// Measurement kernels: the attainable streaming copy and read bandwidth, and
+// the rate of random 64-byte bucket reads (the FIND-OR-PUT's access), each
+// with known bytes per launch, so the hardware counters' byte accounting can
+// be checked against them.
 typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
 
+// Four independent 16-byte loads in flight per thread; NT: non-temporal
+// (streaming) hints on loads and stores.
+template <bool NT>
 __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
   const v4u32 *s = reinterpret_cast<const v4u32 *>(src);
   v4u32 *d = reinterpret_cast<v4u32 *>(dst);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
-    const v4u32 a = __builtin_nontemporal_load(s + i);
-    const v4u32 b = __builtin_nontemporal_load(s + i + stride);
-    const v4u32 c = __builtin_nontemporal_load(s + i + 2 * stride);
-    const v4u32 e = __builtin_nontemporal_load(s + i + 3 * stride);
-    __builtin_nontemporal_store(a, d + i);
-    __builtin_nontemporal_store(b, d + i + stride);
-    __builtin_nontemporal_store(c, d + i + 2 * stride);
-    __builtin_nontemporal_store(e, d + i + 3 * stride);
+    v4u32 x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = NT ? __builtin_nontemporal_load(s + i + k * stride) : s[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (NT)
+        __builtin_nontemporal_store(x[k], d + i + k * stride);
+      else
+        d[i + k * stride] = x[k];
+    }
   }
-  for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+  for (; i < n; i += stride) {
+    if (NT)
+      __builtin_nontemporal_store(__builtin_nontemporal_load(s + i), d + i);
+    else
+      d[i] = s[i];
+  }
 }
 
-hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, src, dst, n);
+// Streaming read of n 16-byte elements; the XOR of everything goes to *sink
+// only if it equals a value no input produces in practice (keeps the loads).
+__global__ __launch_bounds__(256) void k_read(const uint4 *__restrict__ src, uint64_t n, uint32_t *sink) {
+  const v4u32 *s = reinterpret_cast<const v4u32 *>(src);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  v4u32 acc = {0, 0, 0, 0};
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    v4u32 x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = __builtin_nontemporal_load(s + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc ^= x[k];
+  }
+  for (; i < n; i += stride) acc ^= __builtin_nontemporal_load(s + i);
+  const uint32_t r = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (r == 0x9e3779b9u) *sink = r;
+}
+
+// n random 64-byte bucket reads from a table of nbuckets buckets, four lanes
+// per bucket (16 bytes each), as the FIND-OR-PUT's lane groups read them;
+// bucket of lookup i: the FIND-OR-PUT's bucket_of on a hash of i.
+__global__ __launch_bounds__(256) void k_random_read(const uint64_t *__restrict__ table, uint64_t nbuckets,
+                                                      uint64_t n, uint64_t seed, uint32_t *sink) {
+  const int lane = lane_id();
+  const int sub = lane & 3;
+  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  uint64_t acc = 0;
+  for (uint64_t j = gw * 16 + (lane >> 2); j < n; j += nw * 16) {
+    const uint64_t b = bucket_of(mix64(j ^ seed), 0, nbuckets);
+    const ulonglong2 q = *reinterpret_cast<const ulonglong2 *>(table + b * 8 + 2 * sub);
+    acc ^= q.x + q.y;
+  }
+  if (static_cast<uint32_t>(acc) == 0x9e3779b9u) *sink = static_cast<uint32_t>(acc >> 32);
+}
+
+hipError_t launch_copy(const uint4 *src, uint4 *dst, uint64_t n, uint32_t blocks, bool nt, hipStream_t s) {
+  if (nt)
+    hipLaunchKernelGGL(k_copy<true>, dim3(blocks), dim3(256), 0, s, src, dst, n);
+  else
+    hipLaunchKernelGGL(k_copy<false>, dim3(blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_read(const uint4 *src, uint64_t n, uint32_t *sink, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_read, dim3(blocks), dim3(256), 0, s, src, n, sink);
+  return hipGetLastError();
+}
+
+hipError_t launch_random_read(const uint64_t *table, uint64_t nbuckets, uint64_t n, uint64_t seed, uint32_t *sink,
+                              uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_random_read, dim3(blocks), dim3(256), 0, s, table, nbuckets, n, seed, sink);
   return hipGetLastError();
 }
 

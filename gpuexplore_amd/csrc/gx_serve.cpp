@@ -54,6 +54,7 @@ struct Job {
   std::vector<uint32_t> words;  // compiled model
   ModelHdr hdr;
   int32_t property;
+  uint32_t mode, por_method;    // ExpandMode and GX_POR_* (gx_job_options)
   uint64_t pages;               // pages requested
   double t_submit;
   int32_t restarts;
@@ -94,13 +95,14 @@ extern "C" void gx_server_options_default(gx_server_options *o) {
   memset(o, 0, sizeof *o);
   o->slots = 256;
   o->key_words = 2;
-  o->max_procs = 16;
+  o->max_procs = 32;
   o->threads = 512;
   o->rounds_per_step = 16;  // a 10 s stream at 20 jobs/s: p50 0.74 ms (4: 0.98 ms, 1: 1.55 ms)
   o->table_bytes = 16ull << 30;
   o->page_bytes = 4ull << 20;
   o->model_words = 1u << 15;
   o->hash_seed = 0x5eed5eed5eedull;
+  o->por_jobs = 1;
 }
 
 extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
@@ -115,10 +117,12 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   if (o.key_words < 1 || o.key_words > static_cast<uint32_t>(kMaxKeyWords)) return GX_E_ARG;
   if (o.max_procs < 1 || o.max_procs > static_cast<uint32_t>(kMaxProcs)) return GX_E_ARG;
   o.threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (o.threads ? o.threads : 512) / 64 * 64));
-  if (static_cast<size_t>(o.threads / 64) * o.max_procs * 256 > 64 * 1024) return GX_E_LIMIT;
   if (!o.rounds_per_step) o.rounds_per_step = 1;
   if (!o.model_words) o.model_words = 1u << 15;
   o.model_words = (o.model_words + 3u) & ~3u;  // 16-byte aligned model regions
+  // a model has fewer rows than words: 16-bit row indices when models are small
+  const bool rows16 = o.model_words < 0xFFFFu;
+  if (batch_expand_lds(o.threads, o.max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
   s->o = o;
   s->W = static_cast<int>(o.key_words);
   const uint32_t bb = o.key_words <= 2 ? 64u : 128u;
@@ -135,8 +139,11 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   if (!s->o.blocks) s->o.blocks = prop.multiProcessorCount * (2048 / s->o.threads);
   const uint64_t S = o.slots, W = s->W;
   const uint64_t front_total = s->npages * s->page_entries;
+  // successor buffer: a quarter of the frontier pool, 2^24..2^30 entries by
+  // default (a level that does not fit is expanded over several rounds)
   const uint64_t cand_cap = std::min<uint64_t>(
-      0xFFFFFFFFull, o.cand_capacity ? o.cand_capacity : std::max<uint64_t>(1u << 24, front_total));
+      0xFFFFFFFFull,
+      o.cand_capacity ? o.cand_capacity : std::min<uint64_t>(1ull << 30, std::max<uint64_t>(1u << 24, front_total / 4)));
   const uint64_t max_chunks = front_total / 64 + S + 1;
   SV_TRY(s->inst.alloc(sizeof(BatchInst) * S));
   SV_TRY(s->ctrl.alloc(sizeof(BatchCtrl)));
@@ -168,7 +175,13 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   a.deferred[1] = s->def1.as<uint64_t>();
   a.table = s->table.as<uint64_t>();
   a.seed = o.hash_seed;
+  a.bucket_words = static_cast<uint32_t>(s->BW);
   a.max_procs = o.max_procs;
+  a.rows16 = rows16 ? 1u : 0u;
+  a.por = o.por_jobs ? 1u : 0u;
+  a.stamps = o.por_jobs ? 1u : 0u;
+  a.por_threads = 256;
+  a.por_blocks = static_cast<uint32_t>(prop.multiProcessorCount) * 4;
   // capture rounds_per_step rounds once; replays reuse the fixed pools
   SV_TRY(hipStreamSynchronize(s->st));
   SV_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
@@ -195,17 +208,30 @@ extern "C" void gx_server_free(gx_server *s) { delete s; }
 
 extern "C" int gx_server_submit(gx_server *s, const gx_model *m, int32_t property, uint64_t est_states,
                                 uint64_t *job_id) {
-  if (!s || !m || m->por) return GX_E_ARG;
-  if (property < GX_PROP_NONE || property > GX_PROP_LIVENESS) return GX_E_ARG;
+  if (!m || m->por) return GX_E_ARG;  // plain method only (gx_server_submit_job has POR)
+  const gx_job_options j{property, 0, 0, GX_POR_CAMPLE};
+  return gx_server_submit_job(s, m, &j, est_states, job_id);
+}
+
+extern "C" int gx_server_submit_job(gx_server *s, const gx_model *m, const gx_job_options *job,
+                                    uint64_t est_states, uint64_t *job_id) {
+  if (!s || !m || !job) return GX_E_ARG;
+  if (job->property < GX_PROP_NONE || job->property > GX_PROP_LIVENESS || job->por_method > GX_POR_STUBBORN)
+    return GX_E_ARG;
+  if (job->por && !s->o.por_jobs) return GX_E_ARG;
   const ModelHdr &H = m->comp.hdr;
-  if (!H.lab64 || H.key_words > static_cast<uint32_t>(s->W) || H.nprocs > s->o.max_procs ||
+  if ((!job->por && !H.lab64) || H.key_words > static_cast<uint32_t>(s->W) || H.nprocs > s->o.max_procs ||
       m->comp.words.size() > s->model_words)
     return GX_E_LIMIT;
+  // the level stamp of the cycle proviso follows the state bits in the slot
+  if (job->por && job->cycle_proviso && H.state_bits + 32 > 63u * static_cast<uint32_t>(s->W)) return GX_E_LIMIT;
   Job j;
   j.id = s->next_id++;
   j.words = m->comp.words;
   j.hdr = H;
-  j.property = property;
+  j.property = job->property;
+  j.mode = batch_mode(job->por, job->cycle_proviso);
+  j.por_method = job->por_method;
   // footprint: slots for 2x the estimate, at least one page
   const uint64_t slots = std::max<uint64_t>(2 * std::max<uint64_t>(est_states, 1), 1);
   j.pages = std::max<uint64_t>(1, (slots + s->page_entries - 1) / s->page_entries);
@@ -217,9 +243,22 @@ extern "C" int gx_server_submit(gx_server *s, const gx_model *m, int32_t propert
   return GX_OK;
 }
 
-// first fit of a run of `n` free pages
+// A run of `n` free pages: small partitions by first fit from the bottom of
+// the pool, large ones (above 1/16 of it) by first fit from the top, so the
+// many small jobs do not cut the pool into runs too short for a large one.
 static bool alloc_pages(gx_server *s, uint64_t n, uint64_t *first) {
   uint64_t run = 0;
+  if (n * 16 > s->npages) {
+    for (uint64_t q = s->npages; q-- > 0;) {
+      run = s->page_used[q] ? 0 : run + 1;
+      if (run == n) {
+        *first = q;
+        for (uint64_t r = q; r < q + n; r++) s->page_used[r] = 1;
+        return true;
+      }
+    }
+    return false;
+  }
   for (uint64_t p = 0; p < s->npages; p++) {
     run = s->page_used[p] ? 0 : run + 1;
     if (run == n) {
@@ -237,6 +276,8 @@ static int admit(gx_server *s, uint32_t slot, Job &&j, uint64_t page0) {
   I.hdr = j.hdr;
   I.model_off = static_cast<uint32_t>(slot * s->model_words);
   I.property = j.property;
+  I.mode = j.mode;
+  I.por_method = j.por_method;
   I.tb = page0 * s->page_buckets;
   I.nb = j.pages * s->page_buckets;
   I.fo = page0 * s->page_entries;

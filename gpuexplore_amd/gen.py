@@ -441,19 +441,20 @@ def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, st
     return out
 
 
-def config5_job(seed: int) -> Tuple[Network, str, int]:
+def config5_job(seed: int) -> Tuple[Network, str, int, bool]:
     """One job of the serving stream: 90% drawn from the config-3 generator
-    (phased rounds, 8 local states) with 6..12 processes, 10% from the
-    config-2 generator (16 local states, token budgets) with 12..14
-    processes.  Returns (network, flag, estimated states); the estimate is
-    deliberately rough (the server restarts a job whose partition fills)."""
+    (24 processes, 8 local states, phased rounds) and checked with
+    ``--por -d`` -- cluster-based POR without the cycle proviso, since -d
+    switches it off (cu:1851-1874) -- and 10% from the config-2 generator at
+    full scale (20 processes, 16 local states, token budgets, -d).  Returns
+    (network, flag, estimated states, por).  The POR jobs' reduced spaces
+    hold 7e4..4e6 states (mean about 1e6, 40 seeds), the config-2 jobs about
+    3e8; the estimates are what a client would guess, and the server
+    restarts a job whose partition fills."""
     rng = Rng(0xC5 * 1000003 + seed)
     if rng.below(10) == 0:
-        n = 12 + rng.below(3)
-        budgets = [2 if p % 5 == 0 else 1 for p in range(n)]
-        return rounds(n, 16, 4, budgets, 4, seed), "-d", int(2.5 ** n)
-    n = 6 + rng.below(7)
-    return rounds(n, 8, 4, [1] * n, 4, seed, mix=(0.7, 0.2, 0.1), phased=True), "-d", int(3.0 ** n)
+        return config2(seed), "-d", 350_000_000, False
+    return rounds(24, 8, 4, [1] * 24, 4, seed, mix=(0.7, 0.2, 0.1), phased=True), "-d", 2_000_000, True
 
 
 # ----------------------------------------------------------------------

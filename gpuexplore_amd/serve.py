@@ -1,8 +1,9 @@
 """Serving workload: a Poisson stream of model-checking jobs through gx_server.
 
-Jobs come from :func:`gpuexplore_amd.gen.config5_job` (90% phased-rounds
-models with 6..12 processes, 10% token-budget rounds models with 12..14
-processes), each with a deadlock check and a rough footprint estimate.
+Jobs come from :func:`gpuexplore_amd.gen.config5_job`: 90% are 24-process
+phased-rounds models checked with ``--por -d`` (cluster-based POR), 10% are
+20-process token-budget models at the large plain workload's scale (about
+3e8 states each) checked with -d; each comes with a footprint estimate.
 Arrival times are drawn up front; the driver submits every job whose arrival
 time has passed, steps the server, and polls results.  Time-to-verdict is
 measured from a job's scheduled arrival, so time a job spends waiting for the
@@ -32,9 +33,9 @@ from gpuexplore_amd import gen
 def make_jobs(count: int, seed0: int = 0) -> List[dict]:
     jobs = []
     for k in range(count):
-        net, flag, est = gen.config5_job(seed0 + k)
+        net, flag, est, por = gen.config5_job(seed0 + k)
         jobs.append({"seed": seed0 + k, "gpf": net.to_gpf(), "prop": 1 if flag == "-d" else 2,
-                     "est": est, "procs": len(net.nstates)})
+                     "est": est, "procs": len(net.nstates), "por": por})
     return jobs
 
 
@@ -54,13 +55,19 @@ def pct(xs: List[float], q: float) -> float:
     return s[min(len(s) - 1, max(0, int(math.ceil(q * len(s))) - 1))]
 
 
+# Server pools for the stream: a 48 GiB visited set (frontier pools of the
+# same size), enough for four of the 3e8-state jobs at their estimate beside
+# the POR jobs.
+SERVER_KW = {"table_bytes": 48 << 30, "page_bytes": 4 << 20}
+
+
 def run_stream(jobs: List[dict], times: Optional[List[float]], server_kw: Optional[dict] = None,
                log=None) -> Dict:
     """Runs the jobs through one server.  times=None submits everything at
     once (saturation throughput).  Returns per-job results and the stream's
     figures."""
-    models = [gx.Model.from_text(j["gpf"]) for j in jobs]
-    srv = gx.Server(**(server_kw or {}))
+    models = [gx.Model.from_text(j["gpf"], por=j["por"]) for j in jobs]
+    srv = gx.Server(**(server_kw or SERVER_KW))
     n = len(jobs)
     sched = times if times is not None else [0.0] * n
     jid_of: Dict[int, int] = {}
@@ -72,7 +79,7 @@ def run_stream(jobs: List[dict], times: Optional[List[float]], server_kw: Option
     while len(res) < n:
         now = time.monotonic() - t0
         while nxt < n and sched[nxt] <= now:
-            jid = srv.submit(models[nxt], jobs[nxt]["prop"], jobs[nxt]["est"])
+            jid = srv.submit(models[nxt], jobs[nxt]["prop"], jobs[nxt]["est"], por=jobs[nxt]["por"])
             jid_of[jid] = nxt
             late[nxt] = (time.monotonic() - t0) - sched[nxt]  # driver lag before submit
             nxt += 1
@@ -102,11 +109,11 @@ def plain_path(jobs: List[dict]) -> List[dict]:
     service time includes every attempt."""
     out = []
     for j in jobs:
-        m = gx.Model.from_text(j["gpf"])
+        m = gx.Model.from_text(j["gpf"], por=j["por"])
         q = 1 << max(20, (8 * j["est"]).bit_length())  # 32-bit ints: >= 4 per estimated state
         t = time.monotonic()
         while True:
-            r = gx.explore(m, property=j["prop"], q_ints=q)
+            r = gx.explore(m, property=j["prop"], por=int(j["por"]), q_ints=q)
             if not r["table_full"] or q >= (1 << 32):
                 break
             q *= 4
@@ -174,7 +181,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--rounds-per-step", type=int, default=16)
-    ap.add_argument("--table-gib", type=float, default=16.0)
+    ap.add_argument("--table-gib", type=float, default=48.0)
     args = ap.parse_args(argv)
     kw = {"slots": args.slots, "rounds_per_step": args.rounds_per_step,
           "table_bytes": int(args.table_gib * (1 << 30))}

@@ -157,6 +157,16 @@ void gx_explorer_free(gx_explorer *e);
 const char *gx_strerror(int status);
 
 /* ------------------------------------------------------------------ */
+/* Per-job options of a batch instance or a served job: the reference's -d/-p
+ * --por --cycle-proviso (cu:1785-1883) for one model, as in gx_options. */
+typedef struct gx_job_options {
+  int32_t property;        /* GX_PROP_* */
+  int32_t por;             /* partial-order reduction */
+  int32_t cycle_proviso;   /* with por: the cycle proviso */
+  uint32_t por_method;     /* GX_POR_* with por */
+} gx_job_options;
+
+/* ------------------------------------------------------------------ */
 /* Batched exploration: n independent models (plain method, each with its own
  * property) explored together.  Every instance owns a partition of one
  * visited set and its own frontiers; one round expands one BFS level of
@@ -213,7 +223,7 @@ typedef struct gx_server_options {
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint64_t hash_seed;
   uint32_t model_words;      /* largest compiled model accepted (32-bit words) */
-  uint32_t reserved;
+  uint32_t por_jobs;         /* 1: POR jobs accepted (each round also runs the POR kernel) */
 } gx_server_options;
 
 typedef struct gx_server_status {
@@ -236,6 +246,11 @@ void gx_server_options_default(gx_server_options *o);
 int gx_server_create(const gx_server_options *o, gx_server **out);
 int gx_server_submit(gx_server *s, const gx_model *m, int32_t property, uint64_t est_states,
                      uint64_t *job_id);
+/* Same with the job's options: POR jobs need por_jobs in the server options
+ * (GX_E_ARG otherwise); with the cycle proviso, state_bits + 32 must fit the
+ * pool's key_words (GX_E_LIMIT otherwise). */
+int gx_server_submit_job(gx_server *s, const gx_model *m, const gx_job_options *job, uint64_t est_states,
+                         uint64_t *job_id);
 int gx_server_step(gx_server *s, gx_server_status *status);
 int gx_server_poll(gx_server *s, gx_job_result *out, int max, int *n);
 void gx_server_free(gx_server *s);
@@ -246,6 +261,17 @@ void gx_server_free(gx_server *s);
 int gx_explore_batch(const gx_model *const *models, const int32_t *properties,
                      const uint64_t *weights, int n, const gx_batch_options *o,
                      gx_result *results, gx_batch_stats *stats);
+
+/* Same with options per instance: plain instances run the state-per-lane
+ * kernel, POR instances (cample, ample or stubborn, with or without the cycle
+ * proviso; the reference's gather_por, cu:1191-1733) the process-group
+ * kernel, both in every round.  Per instance the result equals gx_explore's
+ * with the same options.  With the cycle proviso a slot carries a 32-bit
+ * level stamp after the state bits, so the batch's key width grows to
+ * ceil((state_bits + 32) / 63) words where needed. */
+int gx_explore_batch_jobs(const gx_model *const *models, const gx_job_options *jobs,
+                          const uint64_t *weights, int n, const gx_batch_options *o,
+                          gx_result *results, gx_batch_stats *stats);
 
 /* ------------------------------------------------------------------ */
 /* Visited-set access (FIND-OR-PUT on its own), the hash-insert phase of the
@@ -296,9 +322,17 @@ int gx_device_free(void *dev_ptr);
 int gx_copy_to_device(void *dev_dst, const void *host_src, uint64_t bytes);
 int gx_copy_to_host(void *host_dst, const void *dev_src, uint64_t bytes);
 
-/* Streaming copy of `bytes` bytes device-to-device, timed with HIP events:
- * the measured peak the hash phase is compared against. */
+/* Measured bandwidth figures beside the published HBM peak, timed with HIP
+ * events over `reps` launches (best launch shape of a small sweep):
+ * gx_bench_copy   streaming copy of `bytes` bytes device to device (GB/s,
+ *                 bytes read + bytes written);
+ * gx_bench_read   streaming read of `bytes` bytes (GB/s);
+ * gx_bench_random `lookups` reads of random 64-byte buckets of a table of
+ *                 `table_bytes` bytes, four lanes of 16 bytes per bucket as
+ *                 the FIND-OR-PUT reads them (lookups per second). */
 int gx_bench_copy(uint64_t bytes, int reps, double *gbps);
+int gx_bench_read(uint64_t bytes, int reps, double *gbps);
+int gx_bench_random(uint64_t table_bytes, uint64_t lookups, int reps, double *lookups_per_s);
 
 #ifdef __cplusplus
 }
