@@ -10,8 +10,11 @@ from the initial state (the visited set is reset at the start of the step;
 device buffers are allocated once, before the timed steps).
 
 Prints one JSON line: new states/s (whole job), the FIND-OR-PUT kernel's
-achieved algorithmic bandwidth against the HBM peak, and a CPU baseline (the
-oracle's BFS on a bounded sample of the same workload, one host thread).
+achieved algorithmic bandwidth against the HBM peak (and, from the committed
+rocprofv3 counter summary of the same command, the bytes it moved to and
+from memory and its atomic rate), and a CPU baseline: the oracle's BFS of the
+whole workload on the host cores one GPU's share allows (16), and a bounded
+sample of it on one core.
 """
 
 from __future__ import annotations
@@ -28,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E, published
 BYTES_PER_FINDORPUT = 128  # one 128-byte bucket line per lookup, one per insert
+GOLDEN = os.path.join(ROOT, "tests", "golden", "config2_counts.json")
+CPU_CORES = 16  # the host share of one GPU
 
 
 def batched_vs_plain(gx, gen, tmp: str) -> dict:
@@ -114,7 +119,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--q-log2", type=int, default=34, help="visited set size: 2^q 32-bit ints")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample-states", type=int, default=10_000_000)
+    ap.add_argument("--cpu-sample-states", type=int, default=0,
+                    help="multi-threaded CPU baseline: stop after the level reaching this many states (0: all)")
+    ap.add_argument("--cpu-single-states", type=int, default=10_000_000,
+                    help="one-thread CPU baseline: stop after the level reaching this many states")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane, 3 fused level")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
@@ -146,6 +154,10 @@ def main() -> int:
     import ctypes
     peak = ctypes.c_double(0)
     gx.lib().gx_bench_copy(4 << 30, 20, ctypes.byref(peak))
+    peak_read = ctypes.c_double(0)
+    gx.lib().gx_bench_read(4 << 30, 20, ctypes.byref(peak_read))
+    rand64 = ctypes.c_double(0)
+    gx.lib().gx_bench_random(32 << 30, 1 << 29, 5, ctypes.byref(rand64))
 
     ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE,
                      expand_kernel=args.expand_kernel, bucket_bytes=args.bucket_bytes, threads=args.threads,
@@ -165,6 +177,16 @@ def main() -> int:
     if any(r["states"] != states or r["transitions"] != runs[0]["transitions"] for r in runs):
         print("state counts differ between steps", file=sys.stderr)
         return 1
+    golden = None
+    if args.seed == 1 and args.q_log2 == 34:
+        # the oracle's committed full-size counts of this exact model
+        g = json.load(open(GOLDEN))["runs"]["plain"]
+        got = (states, runs[0]["transitions"], runs[0]["levels"], bool(runs[0]["violation"]))
+        want = (g["states"], g["transitions"], g["levels"], g["violation"])
+        if got != want:
+            print(f"counts {got} differ from the oracle's {want}", file=sys.stderr)
+            return 1
+        golden = "equal to tests/golden/config2_counts.json (oracle)"
     t_exp = sum(r["t_explore_s"] for r in runs)
     t_hash = sum(r["t_hash_s"] for r in runs)
     lookups = sum(r["lookups"] for r in runs)
@@ -177,7 +199,7 @@ def main() -> int:
     # HBM traffic per k_insert launch from the committed rocprofv3 --pmc runs of
     # this same command (FETCH_SIZE and WRITE_SIZE, scaled by the counters'
     # reading of a streaming copy of known size), when the workload matches
-    traffic = None
+    traffic = physical = atomics = None
     prof = os.path.join(ROOT, "profiles", "k_insert_config2_traffic.json")
     if os.path.exists(prof) and args.q_log2 == 34 and args.seed == 1 and not args.bucket_bytes \
             and not args.expand_kernel:
@@ -185,6 +207,8 @@ def main() -> int:
         for k, v in pj["kernels"].items():
             if "k_insert" in k:
                 traffic = v["bytes_per_launch"]
+                physical = v.get("dram_bytes_per_launch")
+                atomics = v.get("atomics_per_launch")
 
     batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp)
     por = None
@@ -197,30 +221,36 @@ def main() -> int:
         from gpuexplore_amd import serve
         serving = serve.serve_workload(args.serve_rate, args.serve_seconds, seed=0,
                                        log=lambda m: print(m, file=sys.stderr, flush=True))
-        serving["workload"] = "config5: Poisson stream of gen.config5_job models, deadlock check"
+        serving["workload"] = ("config5: Poisson stream of gen.config5_job models: 90% 24-process --por -d, "
+                               "10% 20-process 3e8-state -d (the large plain workload's generator)")
 
     cpu = None
     if not args.no_cpu:
-        # the oracle's BFS on this host: on all the cores this process may use
-        # (at most 16, the host share of one GPU) and on one core, over the
-        # same bounded sample of the workload
+        # the oracle's BFS on this host: the whole workload on the cores one
+        # GPU's share of the host allows (16, whatever the machine has), and a
+        # bounded sample of it (the first levels) on one core
         from oracle import oracle
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        threads = max(1, min(CPU_CORES, len(os.sched_getaffinity(0))))
         om = oracle.OracleModel(path, False)
-        mt = om.explore_mt(threads, oracle.PROP_NONE, max_states=args.cpu_sample_states)
-        st = om.explore(oracle.PROP_NONE, max_states=args.cpu_sample_states)
-        if (mt["states"], mt["transitions"]) != (st["states"], st["transitions"]):
-            print("CPU baselines disagree", file=sys.stderr)
+        full_cpu = args.cpu_sample_states == 0
+        mt = om.explore_mt(threads, oracle.PROP_NONE, max_states=args.cpu_sample_states,
+                           expected_states=states if full_cpu else args.cpu_sample_states)
+        st = om.explore(oracle.PROP_NONE, max_states=args.cpu_single_states)
+        if full_cpu and (mt["states"], mt["transitions"]) != (states, runs[0]["transitions"]):
+            print("CPU baseline counts differ from the GPU's", file=sys.stderr)
             return 1
+        what = "the whole workload" if full_cpu else f"its first {mt['levels']} levels"
         cpu = {
             "value": mt["states"] / mt["seconds"],
             "unit": "new states/s",
             "cores": threads,
             "kind": "port",
-            "sample": (f"oracle BFS of the same model, first {mt['levels']} levels "
-                       f"({mt['states']} states, {mt['transitions']} transitions): {mt['seconds']:.1f} s on "
-                       f"{threads} threads, {st['seconds']:.1f} s on one"),
-            "single_thread": {"value": st["states"] / st["seconds"], "cores": 1, "seconds": st["seconds"]},
+            "sample": (f"oracle BFS (orc_explore_mt, sharded presized visited set) of {what}: "
+                       f"{mt['states']} states, {mt['transitions']} transitions, {mt['levels']} levels in "
+                       f"{mt['seconds']:.1f} s on {threads} threads (host cores {os.cpu_count()}); "
+                       f"one thread: first {st['levels']} levels ({st['states']} states) in {st['seconds']:.1f} s"),
+            "single_thread": {"value": st["states"] / st["seconds"], "cores": 1, "seconds": st["seconds"],
+                              "states": st["states"]},
         }
 
     line = {
@@ -244,6 +274,7 @@ def main() -> int:
             "states": states,
             "transitions": runs[0]["transitions"],
             "levels": runs[0]["levels"],
+            "counts_check": golden,
             "key_words": info["key_words"],
             "bucket_bytes": args.bucket_bytes,
             "threads": args.threads,
@@ -258,10 +289,18 @@ def main() -> int:
             "frac": achieved / PEAK_HBM_GBPS,
             "traffic": traffic,
             "traffic_source": "profiles/k_insert_config2_traffic.json" if traffic else None,
+            # bytes the kernel's L2 misses and write-backs moved to and from
+            # memory (MALL or HBM), from the counters, over its launch time
+            "physical_gbps": physical / (t_hash / launches) / 1e9 if physical else None,
+            "physical_frac": physical / (t_hash / launches) / 1e9 / PEAK_HBM_GBPS if physical else None,
             "kernel": "k_insert (FIND-OR-PUT)",
             "algorithmic_bytes_per_launch": algo_bytes / launches,
             "avg_launch_us": 1e6 * t_hash / launches,
             "measured_copy_peak_gbps": peak.value,
+            "measured_read_peak_gbps": peak_read.value,
+            "measured_random64_lookups_per_s": rand64.value,
+            "lookups_per_s": lookups / t_hash,
+            "lookups_vs_random64": lookups / t_hash / rand64.value if rand64.value else None,
         },
         "hash_phase": {
             "lookups_per_s": lookups / t_hash,
@@ -269,6 +308,9 @@ def main() -> int:
             # each insert claims its slot with one 64-bit atomicCAS; attempts
             # lost to a racing insert into the same slot are not counted
             "cas_per_s": inserts / t_hash,
+            # every L2 atomic of the kernel (CAS claims, frontier and counter
+            # adds), TCC_ATOMIC from the committed counter summary
+            "atomics_per_s": atomics / (t_hash / launches) if atomics else None,
             "t_hash_s_per_step": t_hash / args.steps,
             "t_expand_s_per_step": sum(r["t_expand_s"] for r in runs) / args.steps,
         },

@@ -116,6 +116,8 @@ class ServerOptions(ctypes.Structure):
         ("hash_seed", ctypes.c_uint64),
         ("model_words", ctypes.c_uint32),
         ("por_jobs", ctypes.c_uint32),
+        ("insert_blocks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

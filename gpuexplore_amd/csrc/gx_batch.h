@@ -56,7 +56,7 @@ struct BatchArgs {
   const uint32_t *models;
   uint64_t *front[2];   // pools, W words per entry
   uint64_t *cand;
-  uint32_t *cand_inst;
+  uint32_t *cand_blk;   // instance of each 64-entry block of the candidate buffer
   uint64_t cand_cap;
   uint64_t *deferred[2];  // chunk descriptors: instance << 40 | first state
   uint64_t *table;
@@ -67,6 +67,8 @@ struct BatchArgs {
   uint32_t por;      // POR instances may run: the POR expand kernel is part of the round
   uint32_t stamps;   // instances with the cycle proviso may run: slots carry level stamps
   uint32_t por_blocks, por_threads;  // launch of the POR expand kernel
+  uint32_t chunk_states;  // states per plain workgroup chunk: 64 x the plain expand kernel's waves
+  uint32_t expand_lds;    // dynamic LDS bytes of the plain expand kernel (batch_expand_lds)
 };
 
 // Mode of an instance from its job options (gx_job_options).
@@ -78,13 +80,24 @@ inline uint32_t batch_mode(int por, int cycle_proviso) {
 // unfinished instance (one BFS level)
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
                           uint32_t ins_blocks, hipStream_t s);
-// LDS bytes of the plain batch expand kernel
-size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16);
+// LDS of the plain batch expand kernel: the waves' row tables for the
+// largest model (at most 64 KiB) plus room to stage one model section of up
+// to kBatchModelLds bytes beside them, capped at 64 KiB
+constexpr size_t kBatchModelLds = 30 * 1024;
+inline size_t batch_rows_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
+  return static_cast<size_t>(threads / 64) * max_procs * 64 * (rows16 ? 2 : 4);
+}
+inline size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
+  const size_t b = batch_rows_lds(threads, max_procs, rows16) + kBatchModelLds;
+  return b < 64 * 1024 ? b : 64 * 1024;
+}
 // the advance step alone (with prefix_only set: chunk prefix after admissions)
 hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s);
 // the initial state into a zeroed partition (first bucket tb, nb buckets)
 // and as entry fo of frontier pool front[0]
 hipError_t launch_place_initial(const BatchArgs &a, int W, int BW, uint64_t tb, uint64_t nb, uint64_t fo,
                                 hipStream_t s);
+// the same for instances [0, n) of a.inst at once
+hipError_t launch_place_initial_all(const BatchArgs &a, int W, uint32_t n, hipStream_t s);
 
 }  // namespace gx

@@ -45,89 +45,192 @@ __device__ __forceinline__ uint32_t find_instance(const BatchInst *I, uint32_t n
   return lo;
 }
 
-size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
-  return static_cast<size_t>(threads / 64) * max_procs * 64 * (rows16 ? 2 : 4);
+constexpr uint32_t kNoInst = 0xFFFFFFFFu;
+
+// Transitions and expanded states of the wave's chunks, added to their
+// instance's counters when the wave moves on to another instance (one
+// atomic per switch, not per chunk: on a single large instance every wave
+// of the device would otherwise add into the same two words per chunk).
+struct InstAcc {
+  uint32_t i = kNoInst;
+  uint64_t trans = 0, expanded = 0;
+  __device__ void flush(BatchInst *inst) {
+    if (i != kNoInst && lane_id() == 0 && (trans || expanded)) {
+      atomicAdd(reinterpret_cast<unsigned long long *>(&inst[i].transitions), static_cast<unsigned long long>(trans));
+      atomicAdd(reinterpret_cast<unsigned long long *>(&inst[i].expanded), static_cast<unsigned long long>(expanded));
+    }
+    trans = expanded = 0;
+  }
+  __device__ void add(BatchInst *inst, uint32_t j, uint64_t t, uint64_t e) {
+    if (j != i) {
+      flush(inst);
+      i = j;
+    }
+    trans += t;
+    expanded += e;
+  }
+};
+
+// A wave's share of the candidate buffer in the batch kernels: the
+// single-model reservation (CandChunk, whole 64-entry blocks from the shared
+// counter) where a wave also moves to a fresh block whenever the instance it
+// emits for changes, so every block holds successors of one instance and
+// cand_blk[block] names it: one id per 64 candidates instead of one per
+// candidate, and the FIND-OR-PUT's 64-candidate steps read one id.
+struct BlockChunk {
+  CandChunk ch;
+  uint32_t inst = 0xFFFFFFFFu;
+  __device__ BlockChunk(uint64_t *c, uint64_t cap, uint64_t *cnt) : ch(c, cap, cnt) {}
+  template <int W>
+  __device__ int reserve(uint32_t *cand_blk, uint32_t i, uint64_t total, uint64_t batches, uint64_t *at) {
+    if (!total) return 1;
+    if (i != inst && ch.left) {  // skip to the next block boundary, releasing the gap
+      const uint64_t gap0 = ((ch.base + 63) & ~63ull) - ch.base;
+      const uint64_t gap = gap0 < ch.left ? gap0 : ch.left;
+      ch.release<W>(ch.base, gap);
+      ch.base += gap;
+      ch.left -= gap;
+    }
+    inst = i;
+    const int ok = ch.reserve<W>(total, batches, at);
+    if (ok == 1) {
+      const uint64_t b1 = (*at + total - 1) >> 6;
+      for (uint64_t b = (*at >> 6) + lane_id(); b <= b1; b += 64) cand_blk[b] = i;
+    }
+    return ok;
+  }
+  template <int W>
+  __device__ void release_all() {
+    ch.release_all<W>();
+  }
+};
+
+// One 64-state chunk of a plain instance by one wave, one state per lane
+// (the state-per-lane code of k_expand_tps).  M/H: the model (staged in LDS
+// or global), rows: the wave's row table.  Returns false if the chunk is
+// deferred (the candidate buffer is full) or dropped (it never fits).
+template <int W, typename RowT>
+__device__ __forceinline__ void batch_chunk(const BatchArgs &a, BatchCtrl *C, BlockChunk &ch, InstAcc &acc,
+                                            uint32_t i, uint64_t start, const uint32_t *M, const ModelHdr &H,
+                                            RowT *rows, uint64_t left, uint64_t *def_out, uint64_t *def_out_count) {
+  const int lane = lane_id();
+  BatchInst *I = &a.inst[i];
+  const uint32_t cur = I->cur;
+  const uint64_t si = start + lane;
+  const bool has = si < I->front_count[cur];
+  if (!ballot(has)) return;
+  uint64_t key[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) key[j] = has ? a.front[cur][(I->fo + si) * W + j] : kFlag;
+  const TpsInfo inf = tps_scan<W>(M, H, key, has, rows, lane, I->property == 2);
+  const bool viol = inf.viol || (I->property == 1 && inf.dead);
+  if (ballot(viol) && lane == 0) I->violation = 1;
+  if (ballot(inf.bad) && lane == 0) I->bad_state = 1;
+  uint64_t total = 0;
+  const uint64_t off = wave_excl_scan(inf.cnt, &total);
+  uint64_t obase = 0;
+  const int ok = ch.reserve<W>(a.cand_blk, i, total, left, &obase);
+  if (ok < 0) {
+    // this chunk can never fit: stop its instance (at the level's end),
+    // not the batch, and do not defer the chunk
+    if (lane == 0) {
+      I->cand_overflow = 1;
+      C->cand_overflow = 1;
+    }
+    return;
+  }
+  if (ok == 0) {
+    if (lane == 0) {
+      const uint64_t at = atomicAdd(reinterpret_cast<unsigned long long *>(def_out_count), 1ull);
+      def_out[at] = (static_cast<uint64_t>(i) << 40) | start;
+    }
+    return;
+  }
+  acc.add(a.inst, i, total, static_cast<uint64_t>(__popcll(ballot(has))));
+  if (has) {
+    uint64_t pos = obase + off;
+    tps_emit<W>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
+#pragma unroll
+      for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+      pos++;
+    });
+  }
 }
 
-// Plain instances: 64-state chunks, one state per lane.
+// Plain instances.  A round's plain work is cut into workgroup chunks of
+// waves x 64 states of one instance (a.chunk_states), so all waves of a
+// workgroup expand the same model: the workgroup stages that model's
+// state-per-lane section into LDS when its instance changes (when it fits
+// beside the row tables), as the single-model kernel does, instead of
+// gathering it through the caches lane by lane.  Deferred chunks (64
+// states, from a round whose candidate buffer filled) are expanded one per
+// wave with the model read from global memory.
 template <int W, typename RowT>
 __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
   extern __shared__ uint32_t smem[];
   BatchCtrl *C = a.ctrl;
   if (C->active == 0) return;
-  RowT *rows = reinterpret_cast<RowT *>(smem) + static_cast<uint32_t>(threadIdx.x >> 6) * (a.max_procs * 64u);
   const uint32_t n = C->n;
   const uint32_t use_def = C->use_deferred;
   const uint32_t dc = C->deferred_cur;
-  const uint64_t nchunks = use_def ? C->deferred_count[dc] : C->total_chunks;
   const uint64_t *def_in = a.deferred[dc];
   uint64_t *def_out = a.deferred[dc ^ 1u];
   uint64_t *def_out_count = &C->deferred_count[dc ^ 1u];
-  const int lane = lane_id();
-  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
-  for (uint64_t c = gw; c < nchunks; c += nw) {
-    uint32_t i;
-    uint64_t start;
-    if (use_def) {
+  const uint32_t wib = threadIdx.x >> 6;
+  const uint32_t wpb = blockDim.x >> 6;
+  BlockChunk ch(a.cand, a.cand_cap, &C->cand_count);
+  InstAcc acc;
+  if (use_def) {
+    const uint64_t nchunks = C->deferred_count[dc];
+    const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+    RowT *rows = reinterpret_cast<RowT *>(smem) + wib * (a.max_procs * 64u);
+    for (uint64_t c = gw; c < nchunks; c += nw) {
       const uint64_t d = def_in[c];
-      i = static_cast<uint32_t>(d >> 40);
-      start = d & ((1ull << 40) - 1);
+      const uint32_t i = static_cast<uint32_t>(d >> 40);
       if (a.inst[i].mode != kModePlain) continue;  // the POR kernel's
-    } else {
-      i = find_instance<false>(a.inst, n, c);
-      start = (c - a.inst[i].chunk_prefix) * 64;
+      const ModelHdr H = a.inst[i].hdr;
+      batch_chunk<W, RowT>(a, C, ch, acc, i, d & ((1ull << 40) - 1), a.models + a.inst[i].model_off, H, rows,
+                           (nchunks - c + nw - 1) / nw, def_out, def_out_count);
     }
-    BatchInst *I = &a.inst[i];
-    const ModelHdr H = I->hdr;
-    const uint32_t *M = a.models + I->model_off;
-    const uint32_t cur = I->cur;
-    const uint64_t cnt_front = I->front_count[cur];
-    const uint64_t si = start + lane;
-    const bool has = si < cnt_front;
-    uint64_t key[W];
-#pragma unroll
-    for (int j = 0; j < W; j++) key[j] = has ? a.front[cur][(I->fo + si) * W + j] : kFlag;
-    const TpsInfo inf = tps_scan<W>(M, H, key, has, rows, lane, I->property == 2);
-    const bool viol = inf.viol || (I->property == 1 && inf.dead);
-    if (ballot(viol) && lane == 0) I->violation = 1;
-    if (ballot(inf.bad) && lane == 0) I->bad_state = 1;
-    uint64_t total = 0;
-    const uint64_t off = wave_excl_scan(inf.cnt, &total);
-    uint64_t obase = 0;
-    const int ok = ch.reserve<W>(total, (nchunks - c + nw - 1) / nw, &obase);
-    if (ok < 0) {
-      // this chunk can never fit: stop its instance (at the level's end),
-      // not the batch, and do not defer the chunk
-      if (lane == 0) {
-        I->cand_overflow = 1;
-        C->cand_overflow = 1;
+  } else {
+    const uint64_t nchunks = C->total_chunks;
+    uint32_t staged = kNoInst;
+    uint32_t i = kNoInst;
+    uint64_t lo = 0, hi = 0;  // chunks [lo, hi) belong to instance i
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {  // workgroup-uniform
+      if (c < lo || c >= hi) {  // the chunk left the current instance: search
+        i = find_instance<false>(a.inst, n, c);
+        lo = a.inst[i].chunk_prefix;
+        hi = lo + (a.inst[i].front_count[a.inst[i].cur] + a.chunk_states - 1) / a.chunk_states;
       }
-      continue;
-    }
-    if (ok == 0) {
-      if (lane == 0) {
-        const uint64_t at = atomicAdd(reinterpret_cast<unsigned long long *>(def_out_count), 1ull);
-        def_out[at] = (static_cast<uint64_t>(i) << 40) | start;
+      const BatchInst *I = &a.inst[i];
+      const ModelHdr Hg = I->hdr;
+      const uint32_t *Mg = a.models + I->model_off;
+      const uint32_t sec = Hg.total_words - Hg.tps_begin;
+      const uint32_t sec_w = (sec + 3u) & ~3u;
+      const size_t rows_b = static_cast<size_t>(wpb) * Hg.nprocs * 64u * sizeof(RowT);
+      const bool lds = static_cast<size_t>(sec_w) * 4u + rows_b <= a.expand_lds;
+      // LDS layout: the staged section and N-process row tables behind it,
+      // or (model not staged) row tables of max_procs processes per wave.
+      // Every wave must be done with the old layout before it changes.
+      const uint32_t layout = lds ? i : kNoInst;
+      if (layout != staged) {
+        __syncthreads();
+        if (lds) stage_words(smem, Mg + Hg.tps_begin, sec);
+        __syncthreads();
+        staged = layout;
       }
-      continue;
-    }
-    if (lane == 0 && total) {
-      atomicAdd(reinterpret_cast<unsigned long long *>(&I->transitions), static_cast<unsigned long long>(total));
-      atomicAdd(reinterpret_cast<unsigned long long *>(&I->expanded),
-                static_cast<unsigned long long>(__popcll(ballot(has))));
-    }
-    if (has) {
-      uint64_t pos = obase + off;
-      tps_emit<W>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
-#pragma unroll
-        for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
-        a.cand_inst[pos] = i;
-        pos++;
-      });
+      const ModelHdr H = lds ? tps_rebased(Hg) : Hg;
+      const uint32_t *M = lds ? smem : Mg;
+      RowT *rows = lds ? reinterpret_cast<RowT *>(smem + sec_w) + wib * (Hg.nprocs * 64u)
+                       : reinterpret_cast<RowT *>(smem) + wib * (a.max_procs * 64u);
+      const uint64_t start = (c - I->chunk_prefix) * a.chunk_states + wib * 64u;
+      batch_chunk<W, RowT>(a, C, ch, acc, i, start, M, H, rows, (nchunks - c + gridDim.x - 1) / gridDim.x,
+                           def_out, def_out_count);
     }
   }
+  acc.flush(a.inst);
   ch.release_all<W>();
 }
 
@@ -169,7 +272,7 @@ __global__ __launch_bounds__(256) void k_batch_expand_por(BatchArgs a) {
   const int lane = lane_id();
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
+  BlockChunk ch(a.cand, a.cand_cap, &C->cand_count);
   for (uint64_t c = gw; c < nchunks; c += nw) {
     uint32_t i;
     uint64_t start;
@@ -209,7 +312,6 @@ __global__ __launch_bounds__(256) void k_batch_expand_por(BatchArgs a) {
     auto emit = [&](uint64_t q, const uint64_t (&sv)[W]) {
 #pragma unroll
       for (int j = 0; j < W; j++) a.cand[q * W + j] = sv[j];
-      a.cand_inst[q] = i;
     };
     auto walk = [&]() {
       for (uint64_t b = start; b < end; b += static_cast<uint64_t>(L.G)) {
@@ -229,7 +331,7 @@ __global__ __launch_bounds__(256) void k_batch_expand_por(BatchArgs a) {
     walk();  // count
     if (ballot(acc.bad) && lane == 0) I->bad_state = 1;
     uint64_t obase = 0;
-    const int ok = ch.reserve<W>(total, (nchunks - c + nw - 1) / nw, &obase);
+    const int ok = ch.reserve<W>(a.cand_blk, i, total, (nchunks - c + nw - 1) / nw, &obase);
     if (ok < 0) {
       if (lane == 0) {
         I->cand_overflow = 1;
@@ -260,25 +362,32 @@ __global__ __launch_bounds__(256) void k_batch_expand_por(BatchArgs a) {
 // appended to that instance's next frontier in blocks, one atomicAdd per
 // block: candidates of one instance come in long runs, and a per-key atomic
 // on one instance's counter would serialise the device.
-constexpr int kBatchStage = 128;
+constexpr int kBatchStage = 256;
 
+// FIND-OR-PUT of every candidate into its instance's partition.  As in
+// k_insert, a wave walks runs of kInsertRun consecutive iterations in steps
+// of 64 candidates (a pass too small for that spreads one iteration at a time
+// over every wave): each lane reads one candidate, resolves its instance's
+// partition (first bucket, size, and with stamps the slot format's state
+// bits and level) and hashes it once, and the step's iterations hand all of
+// that to the lane groups in candidate order.
 template <int W, int BW, bool STAMPS>
-__global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<W, STAMPS>)))
+void k_batch_insert(BatchArgs a) {
   extern __shared__ uint64_t bstage_all[];
   BatchCtrl *C = a.ctrl;
   if (C->active == 0) return;
   const uint64_t n = C->cand_count < a.cand_cap ? C->cand_count : a.cand_cap;
   constexpr int LPB = BW / 2;
   constexpr int CPW = 64 / LPB;
-  SlotFmt<W> fmt;
-  make_fmt<W, false>(fmt, 0, 0);  // every bit compared, no stamp (instances without the proviso)
+  static_assert(CPW <= kBatchStage, "stage smaller than one iteration");
   const int lane = lane_id();
   const int sub = lane % LPB;
   const int cg = lane / LPB;
   const int gbase = cg * LPB;
   uint64_t *stage = bstage_all + static_cast<size_t>(threadIdx.x >> 6) * (kBatchStage * W);
-  uint32_t staged = 0;          // wave-uniform
-  uint32_t stage_inst = 0;      // instance of the staged keys
+  uint32_t staged = 0;      // wave-uniform
+  uint32_t stage_inst = 0;  // instance of the staged keys
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
   uint64_t acc_look = 0;
@@ -304,48 +413,64 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
     wave_sync();
     staged = 0;
   };
-  // runs of consecutive iterations per wave, as in k_insert
-  const uint64_t RUN = CPW * insert_run(n, nw * CPW);
-  for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
-    const uint64_t ci = c0 + cg;
-    c0 += CPW;
-    if (c0 == r0 + RUN) {
-      r0 += nw * RUN;
-      c0 = r0;
-    }
-    bool active = ci < n;
-    uint64_t key[W];
-#pragma unroll
-    for (int j = 0; j < W; j++) key[j] = active ? __builtin_nontemporal_load(a.cand + ci * W + j) : kFlag;
-    active = active && key[0] != 0;
-    const uint32_t i = active ? a.cand_inst[ci] : 0u;
-    const BatchInst *I = &a.inst[i];
-    Table T{a.table + (active ? I->tb : 0) * BW, active ? I->nb : 1, a.seed};
-    int st;
+  // What a candidate's lookup needs besides its key: its instance, the
+  // partition, its first bucket and hash, and (STAMPS) the slot format.
+  struct Cand {
+    uint32_t i, K, stamp;
+    uint64_t tb, nb, h, b0;
+  };
+  // The instance's partition, read once for the whole wave when every
+  // active lane's candidate belongs to one instance (long runs of one
+  // instance are the rule), else per lane.
+  auto part_of = [&](Cand &r, const BatchInst *I, bool active) {
+    r.tb = active ? I->tb : 0;
+    r.nb = active ? I->nb : 1;
     if (STAMPS) {
-      // new states are stamped with the level they are found at
-      SlotFmt<W> f;
-      batch_fmt<W>(f, I, I->level + 1u);
-      uint64_t v[W];
-      slot_value<W, W>(key, f, v);
-      st = find_or_put_group<W, BW>(T, v, f, key_hash<W>(key, a.seed), sub, gbase, active);
-    } else {
-      st = find_or_put_group<W, BW>(T, key, fmt, key_hash<W>(key, a.seed), sub, gbase, active);
+      r.K = (active && I->mode == kModePorProviso) ? I->hdr.state_bits : 63u * W;
+      r.stamp = active ? I->level + 1u : 0u;  // new states are stamped with their level
     }
+  };
+  auto resolve = [&](const uint64_t (&key)[W], uint64_t ci, bool active) {
+    Cand r{};
+    r.i = active ? a.cand_blk[ci >> 6] : kNoInst;
+    const uint64_t am = ballot(active);
+    const uint32_t i0 = am ? static_cast<uint32_t>(__shfl(static_cast<int>(r.i), __ffsll(static_cast<unsigned long long>(am)) - 1, 64))
+                           : 0u;
+    if (!ballot(active && r.i != i0)) {
+      part_of(r, &a.inst[__builtin_amdgcn_readfirstlane(i0)], active);
+    } else {
+      part_of(r, &a.inst[active ? r.i : 0u], active);
+    }
+    if (!active) r.i = 0;
+    r.h = key_hash<W>(key, a.seed);
+    r.b0 = bucket_of(r.h, 0, r.nb);
+    return r;
+  };
+  auto visit = [&](const uint64_t (&key)[W], const Cand &c, bool active) {
+    const Table T{a.table + c.tb * BW, c.nb, a.seed};
+    SlotFmt<W> f;
+    if (STAMPS)
+      make_fmt<W, true>(f, c.K, c.stamp);
+    else
+      make_fmt<W, false>(f, 0, 0);
+    uint64_t v[W];
+    slot_value<W, W>(key, f, v);
+    const uint64_t h = c.h;
+    const int st = find_or_put_group_b<W, BW>(T, v, f, c.b0, [h]() { return h; }, sub, gbase, active);
     acc_look += __popcll(ballot(active && sub == 0));
     if (active && sub == 0) {
-      if (st == kFull) a.inst[i].table_full = 1;
-      if (st == kTimeout) a.inst[i].spin_timeout = 1;
+      if (st == kFull) a.inst[c.i].table_full = 1;
+      if (st == kTimeout) a.inst[c.i].spin_timeout = 1;
     }
     const bool is_new = active && st == kInserted;
     uint64_t m = ballot(is_new && sub == 0);
     while (m) {  // one group per instance present in this iteration
-      const uint32_t i0 = __shfl(i, __ffsll(static_cast<unsigned long long>(m)) - 1, 64);
-      const uint64_t mm = ballot(is_new && sub == 0 && i == i0);
+      const uint32_t i0 = __shfl(c.i, __ffsll(static_cast<unsigned long long>(m)) - 1, 64);
+      const uint64_t mm = ballot(is_new && sub == 0 && c.i == i0);
       const uint32_t cnt = __popcll(mm);
       if (staged && (stage_inst != i0 || staged + cnt > static_cast<uint32_t>(kBatchStage))) flush();
       stage_inst = i0;
-      if (is_new && i == i0) {
+      if (is_new && c.i == i0) {
         const uint32_t pos = staged + __popcll(mm & ((1ull << gbase) - 1ull));
 #pragma unroll
         for (int j = 0; j < W; j++)
@@ -353,6 +478,58 @@ __global__ __launch_bounds__(512) void k_batch_insert(BatchArgs a) {
       }
       staged += cnt;
       m &= ~mm;
+    }
+  };
+  const uint64_t iters = insert_run(n, nw * CPW);
+  if (iters < kInsertSpread) {
+    const uint64_t RUN = CPW * iters;
+    for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
+      const uint64_t ci = c0 + cg;
+      c0 += CPW;
+      if (c0 == r0 + RUN) {
+        r0 += nw * RUN;
+        c0 = r0;
+      }
+      uint64_t key[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) key[j] = ci < n ? __builtin_nontemporal_load(a.cand + ci * W + j) : 0;
+      const bool active = key[0] != 0;  // past the end or a released reservation: zero
+      visit(key, resolve(key, ci, active), active);
+    }
+  } else {
+    static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
+    const uint64_t RUN = CPW * kInsertRun;
+    for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
+      const uint64_t cl = c0 + lane;
+      c0 += 64;
+      if (c0 == r0 + RUN) {
+        r0 += nw * RUN;
+        c0 = r0;
+      }
+      uint64_t key_l[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) key_l[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
+      const bool act_l = key_l[0] != 0;
+      const uint64_t act_m = ballot(act_l);
+      const Cand c_l = resolve(key_l, cl, act_l);
+#pragma unroll 1
+      for (int s2 = 0; s2 < LPB; s2++) {
+        const int src = s2 * CPW + cg;
+        uint64_t key[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) key[j] = __shfl(key_l[j], src, 64);
+        Cand c;
+        c.i = __shfl(c_l.i, src, 64);
+        c.tb = __shfl(c_l.tb, src, 64);
+        c.nb = __shfl(c_l.nb, src, 64);
+        c.h = __shfl(c_l.h, src, 64);
+        c.b0 = __shfl(c_l.b0, src, 64);
+        if (STAMPS) {
+          c.K = __shfl(c_l.K, src, 64);
+          c.stamp = __shfl(c_l.stamp, src, 64);
+        }
+        visit(key, c, (act_m >> src) & 1ull);
+      }
     }
   }
   flush();
@@ -411,7 +588,10 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
       }
       if (!I->stop) {
         const uint64_t f = I->front_count[I->cur];
-        chunks[I->mode == kModePlain ? 0 : 1] = (f + 63) / 64;
+        if (I->mode == kModePlain)
+          chunks[0] = (f + a.chunk_states - 1) / a.chunk_states;
+        else
+          chunks[1] = (f + 63) / 64;
         atomicAdd(&nact, 1u);
       }
     }
@@ -454,7 +634,7 @@ static void launch_insert(const BatchArgs &a, uint32_t blocks, hipStream_t s) {
 template <int W>
 hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, uint32_t ins_blocks,
                         hipStream_t s) {
-  const size_t shm = batch_expand_lds(threads, a.max_procs, a.rows16 != 0);
+  const size_t shm = a.expand_lds;
   if (a.rows16)
     hipLaunchKernelGGL((k_batch_expand<W, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
   else
@@ -469,30 +649,6 @@ hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t th
     case 32: launch_insert<W, 32>(a, ins_blocks, s); break;
     default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
-  return hipGetLastError();
-}
-
-static hipError_t launch_insert_only(const BatchArgs &a, int W, int BW, hipStream_t s) {
-  // the initial states: one candidate per instance
-#define GX_BI(Wk)                                           \
-  case Wk:                                                  \
-    switch (BW) {                                           \
-      case 8: launch_insert<Wk, 8>(a, 64, s); break;        \
-      case 16: launch_insert<Wk, 16>(a, 64, s); break;      \
-      case 32: launch_insert<Wk, 32>(a, 64, s); break;      \
-      default: return hipErrorInvalidValue;                 \
-    }                                                       \
-    break;
-  switch (W) {
-    GX_BI(1)
-    GX_BI(2)
-    GX_BI(3)
-    GX_BI(4)
-    GX_BI(5)
-    default: return hipErrorInvalidValue;
-  }
-#undef GX_BI
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
@@ -525,6 +681,38 @@ __global__ void k_place_initial(BatchArgs a, uint64_t tb, uint64_t nb, uint64_t 
     slot0[j] = key[j];
     a.front[0][fo * W + j] = key[j];
   }
+}
+
+// every instance's initial state, one thread per instance (placed as
+// k_place_initial places one)
+template <int W>
+__global__ void k_place_initial_all(BatchArgs a, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BatchInst &I = a.inst[i];
+  uint64_t key[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) key[j] = kFlag;
+  const uint64_t b = bucket_of(key_hash<W>(key, a.seed), 0, I.nb);
+  uint64_t *slot0 = a.table + (I.tb + b) * a.bucket_words;
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    slot0[j] = key[j];
+    a.front[0][I.fo * W + j] = key[j];
+  }
+}
+
+hipError_t launch_place_initial_all(const BatchArgs &a, int W, uint32_t n, hipStream_t s) {
+  const dim3 g((n + 255) / 256), b(256);
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_place_initial_all<1>, g, b, 0, s, a, n); break;
+    case 2: hipLaunchKernelGGL(k_place_initial_all<2>, g, b, 0, s, a, n); break;
+    case 3: hipLaunchKernelGGL(k_place_initial_all<3>, g, b, 0, s, a, n); break;
+    case 4: hipLaunchKernelGGL(k_place_initial_all<4>, g, b, 0, s, a, n); break;
+    case 5: hipLaunchKernelGGL(k_place_initial_all<5>, g, b, 0, s, a, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_place_initial(const BatchArgs &a, int W, int BW, uint64_t tb, uint64_t nb, uint64_t fo,
@@ -623,8 +811,10 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
     inst[i].property = J.property;
     inst[i].mode = batch_mode(J.por, J.cycle_proviso);
     inst[i].por_method = J.por_method;
-    inst[i].cur = 1;
-    inst[i].level = 0xFFFFFFFFu;
+    inst[i].cur = 0;
+    inst[i].level = 0;
+    inst[i].front_count[0] = 1;  // the initial state, placed at level 0 below
+    inst[i].inserts = 1;
     mbuf.insert(mbuf.end(), m->comp.words.begin(), m->comp.words.end());
     while (mbuf.size() % 4) mbuf.push_back(0);
   }
@@ -637,7 +827,7 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   uint32_t threads = o->threads ? o->threads : 256;
   threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (threads + 63) / 64 * 64));
   const bool rows16 = max_rows < 0xFFFFu;
-  if (batch_expand_lds(threads, max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
+  if (batch_rows_lds(threads, max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
   int dev = 0;
   BX_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -678,7 +868,7 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   BX_TRY(d_f0.alloc(front_total * W * 8));
   BX_TRY(d_f1.alloc(front_total * W * 8));
   BX_TRY(d_cand.alloc(cand_cap * W * 8));
-  BX_TRY(d_cinst.alloc(cand_cap * 4));
+  BX_TRY(d_cinst.alloc((cand_cap / 64 + 1) * 4));
   BX_TRY(d_def0.alloc(max_chunks * 8));
   BX_TRY(d_def1.alloc(max_chunks * 8));
   BX_TRY(d_table.alloc(total_buckets * bb));
@@ -692,17 +882,14 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   BX_TRY(hipMemsetAsync(d_table.p, 0, total_buckets * bb, st));
   BX_TRY(hipMemcpyAsync(d_models.p, mbuf.data(), mbuf.size() * 4, hipMemcpyHostToDevice, st));
   BX_TRY(hipMemcpyAsync(d_inst.p, inst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice, st));
+  // every instance starts at level 0 with its initial state (all processes
+  // in local state 0, cu:668-682) in its partition and frontier; the first
+  // advance only computes the chunk prefixes
   BatchCtrl c0{};
   c0.n = static_cast<uint32_t>(n);
   c0.active = static_cast<uint32_t>(n);
-  c0.cand_count = static_cast<uint64_t>(n);
+  c0.prefix_only = 1;
   BX_TRY(hipMemcpyAsync(d_ctrl.p, &c0, sizeof c0, hipMemcpyHostToDevice, st));
-  // initial states: one all-zero key per instance (cu:668-682)
-  std::vector<uint64_t> s0(static_cast<size_t>(n) * W, kFlag);
-  std::vector<uint32_t> s0i(n);
-  for (int i = 0; i < n; i++) s0i[i] = static_cast<uint32_t>(i);
-  BX_TRY(hipMemcpyAsync(d_cand.p, s0.data(), s0.size() * 8, hipMemcpyHostToDevice, st));
-  BX_TRY(hipMemcpyAsync(d_cinst.p, s0i.data(), s0i.size() * 4, hipMemcpyHostToDevice, st));
   BatchArgs a{};
   a.inst = static_cast<BatchInst *>(d_inst.p);
   a.ctrl = static_cast<BatchCtrl *>(d_ctrl.p);
@@ -710,7 +897,7 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   a.front[0] = static_cast<uint64_t *>(d_f0.p);
   a.front[1] = static_cast<uint64_t *>(d_f1.p);
   a.cand = static_cast<uint64_t *>(d_cand.p);
-  a.cand_inst = static_cast<uint32_t *>(d_cinst.p);
+  a.cand_blk = static_cast<uint32_t *>(d_cinst.p);
   a.cand_cap = cand_cap;
   a.deferred[0] = static_cast<uint64_t *>(d_def0.p);
   a.deferred[1] = static_cast<uint64_t *>(d_def1.p);
@@ -723,7 +910,10 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   a.stamps = any_stamp ? 1u : 0u;
   a.por_threads = 256;
   a.por_blocks = prop.multiProcessorCount * 4;
-  BX_TRY(launch_insert_only(a, W, BW, st));
+  a.chunk_states = threads;  // 64 states per wave
+  a.expand_lds = static_cast<uint32_t>(batch_expand_lds(threads, max_procs, rows16));
+  BX_TRY(launch_place_initial_all(a, W, static_cast<uint32_t>(n), st));
+  BX_TRY(launch_batch_advance(a, st));
   BX_TRY(hipStreamSynchronize(st));
   const double t_init = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_init0).count();
 

@@ -137,6 +137,27 @@ __device__ __forceinline__ uint64_t wave_excl_scan(uint64_t v, uint64_t *total) 
   return x - v;
 }
 
+// Copies n words of the model into LDS with the whole workgroup, eight
+// loads in flight per thread: a loop with one load per iteration waits a
+// memory round trip per 4 B x blockDim, which made a level of one state
+// take 30-58 us in successor generation.
+__device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *__restrict__ src, uint32_t n) {
+  constexpr uint32_t K = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
+    uint32_t v[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t i = i0 + k * blockDim.x;
+      v[k] = i < n ? src[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t i = i0 + k * blockDim.x;
+      if (i < n) dst[i] = v[k];
+    }
+  }
+}
+
 // ordering of LDS traffic between lanes of one wave
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

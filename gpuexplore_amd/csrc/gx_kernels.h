@@ -76,6 +76,14 @@ hipError_t launch_ctrl_reset(Ctrl *c, uint64_t n, hipStream_t s);
 constexpr uint32_t kInsertThreads = 256;
 constexpr uint32_t kInsertBlocksPerCU = 64;
 
+// The FIND-OR-PUT lookups are latency bound, so resident waves matter:
+// eight per SIMD are asked for where slots are at most two words and carry
+// no level stamp (the hashed steps otherwise take about 106 SGPRs and seven
+// waves; asking costs a few SGPR spills to VGPR lanes and runs 4 % faster).
+// Wider or stamped slots would spill VGPRs to scratch, so they ask nothing.
+template <int WT, bool STAMP>
+constexpr int kInsertMinWaves = (WT <= 2 && !STAMP) ? 8 : 1;
+
 // Candidate entries a wave reserves per atomic (at most; see CandChunk).
 // All waves of the device share one counter, so every reservation is a
 // same-address atomic.  With one per 64-state batch (a ring of 20 16-state
@@ -133,6 +141,7 @@ struct CandChunk {
       uint64_t want = total * batches;
       want = want < kChunk ? want : kChunk;
       want = want > total ? want : total;
+      want = (want + 63) & ~63ull;  // whole 64-entry blocks: chunks start block-aligned
       uint64_t got = 0;
       if (lane_id() == 0)
         got = atomicAdd(reinterpret_cast<unsigned long long *>(count), static_cast<unsigned long long>(want));

@@ -22,27 +22,6 @@ namespace gx {
 // 20 16-state processes); 384 no longer fits eight workgroups per CU in LDS.
 constexpr int kStage = 256;
 
-// Copies n words of the model into LDS with the whole workgroup, eight
-// loads in flight per thread: a loop with one load per iteration waits a
-// memory round trip per 4 B x blockDim, which made a level of one state
-// take 30-58 us in successor generation.
-__device__ __forceinline__ void stage_words(uint32_t *dst, const uint32_t *__restrict__ src, uint32_t n) {
-  constexpr uint32_t K = 8;
-  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += K * blockDim.x) {
-    uint32_t v[K];
-#pragma unroll
-    for (uint32_t k = 0; k < K; k++) {
-      const uint32_t i = i0 + k * blockDim.x;
-      v[k] = i < n ? src[i] : 0u;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < K; k++) {
-      const uint32_t i = i0 + k * blockDim.x;
-      if (i < n) dst[i] = v[k];
-    }
-  }
-}
-
 // adds v to counter `which` through the workgroup's partial-counter slot
 __device__ __forceinline__ void add_counter(Ctrl *C, uint32_t which, uint64_t v) {
   if (v)
@@ -401,13 +380,8 @@ hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, boo
 // atomicAdd per block: a single contended counter would otherwise serialise
 // all waves of the device.
 //
-// The lookups are latency bound, so resident waves matter: eight per SIMD
-// are asked for explicitly where slots are at most two words and carry no
-// level stamp (the hashed steps below otherwise take 106 SGPRs, seven waves;
-// asking costs a few SGPR spills to VGPR lanes, and FIND-OR-PUT runs 4 %
-// faster).  Wider slots would spill VGPRs to scratch, so they ask nothing.
-template <int WT, bool STAMP>
-constexpr int kInsertMinWaves = (WT <= 2 && !STAMP) ? 8 : 1;
+// Eight resident waves per SIMD are asked for where slots allow it
+// (kInsertMinWaves).
 
 template <int W, int WT, int BW, bool STAMP>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<WT, STAMP>)))

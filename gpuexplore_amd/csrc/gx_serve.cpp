@@ -68,8 +68,10 @@ struct gx_server {
   uint64_t spb = 4, page_buckets = 0, page_entries = 0, npages = 0, model_words = 0;
   DevBuf inst, ctrl, models, f0, f1, cand, cinst, def0, def1, table;
   hipStream_t st = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // rounds_per_step rounds, captured twice: with level stamps (replayed while
+  // a job with the cycle proviso runs) and without (the leaner FIND-OR-PUT)
+  hipGraph_t graph = nullptr, graph_nostamp = nullptr;
+  hipGraphExec_t exec = nullptr, exec_nostamp = nullptr;
   BatchArgs a{};
   uint64_t next_id = 1;
   std::deque<Job> waiting;
@@ -86,6 +88,8 @@ struct gx_server {
     if (st) (void)hipStreamSynchronize(st);  // nothing may still run on the pools
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (exec_nostamp) (void)hipGraphExecDestroy(exec_nostamp);
+    if (graph_nostamp) (void)hipGraphDestroy(graph_nostamp);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -96,7 +100,7 @@ extern "C" void gx_server_options_default(gx_server_options *o) {
   o->slots = 256;
   o->key_words = 2;
   o->max_procs = 32;
-  o->threads = 512;
+  o->threads = 256;
   o->rounds_per_step = 16;  // a 10 s stream at 20 jobs/s: p50 0.74 ms (4: 0.98 ms, 1: 1.55 ms)
   o->table_bytes = 16ull << 30;
   o->page_bytes = 4ull << 20;
@@ -116,13 +120,13 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   if (o.slots > (1u << 16)) return GX_E_ARG;
   if (o.key_words < 1 || o.key_words > static_cast<uint32_t>(kMaxKeyWords)) return GX_E_ARG;
   if (o.max_procs < 1 || o.max_procs > static_cast<uint32_t>(kMaxProcs)) return GX_E_ARG;
-  o.threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (o.threads ? o.threads : 512) / 64 * 64));
+  o.threads = std::min<uint32_t>(512, std::max<uint32_t>(64, (o.threads ? o.threads : 256) / 64 * 64));
   if (!o.rounds_per_step) o.rounds_per_step = 1;
   if (!o.model_words) o.model_words = 1u << 15;
   o.model_words = (o.model_words + 3u) & ~3u;  // 16-byte aligned model regions
   // a model has fewer rows than words: 16-bit row indices when models are small
   const bool rows16 = o.model_words < 0xFFFFu;
-  if (batch_expand_lds(o.threads, o.max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
+  if (batch_rows_lds(o.threads, o.max_procs, rows16) > 64 * 1024) return GX_E_LIMIT;
   s->o = o;
   s->W = static_cast<int>(o.key_words);
   const uint32_t bb = o.key_words <= 2 ? 64u : 128u;
@@ -137,13 +141,16 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   hipDeviceProp_t prop;
   SV_TRY(hipGetDeviceProperties(&prop, dev));
   if (!s->o.blocks) s->o.blocks = prop.multiProcessorCount * (2048 / s->o.threads);
+  if (!s->o.insert_blocks) s->o.insert_blocks = prop.multiProcessorCount * 16;
   const uint64_t S = o.slots, W = s->W;
   const uint64_t front_total = s->npages * s->page_entries;
-  // successor buffer: a quarter of the frontier pool, 2^24..2^30 entries by
-  // default (a level that does not fit is expanded over several rounds)
+  // successor buffer: as many entries as the frontier pool, 2^24..2^30 by
+  // default.  A level that does not fit is expanded over several rounds; the
+  // largest levels of a 3e8-state job emit about 7e8 successors, so 2^30
+  // keeps them to one round.
   const uint64_t cand_cap = std::min<uint64_t>(
       0xFFFFFFFFull,
-      o.cand_capacity ? o.cand_capacity : std::min<uint64_t>(1ull << 30, std::max<uint64_t>(1u << 24, front_total / 4)));
+      o.cand_capacity ? o.cand_capacity : std::min<uint64_t>(1ull << 30, std::max<uint64_t>(1u << 24, front_total)));
   const uint64_t max_chunks = front_total / 64 + S + 1;
   SV_TRY(s->inst.alloc(sizeof(BatchInst) * S));
   SV_TRY(s->ctrl.alloc(sizeof(BatchCtrl)));
@@ -151,7 +158,7 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   SV_TRY(s->f0.alloc(front_total * W * 8));
   SV_TRY(s->f1.alloc(front_total * W * 8));
   SV_TRY(s->cand.alloc(cand_cap * W * 8));
-  SV_TRY(s->cinst.alloc(cand_cap * 4));
+  SV_TRY(s->cinst.alloc((cand_cap / 64 + 1) * 4));
   SV_TRY(s->def0.alloc(max_chunks * 8));
   SV_TRY(s->def1.alloc(max_chunks * 8));
   SV_TRY(s->table.alloc(s->npages * s->page_buckets * bb));
@@ -169,7 +176,7 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   a.front[0] = s->f0.as<uint64_t>();
   a.front[1] = s->f1.as<uint64_t>();
   a.cand = s->cand.as<uint64_t>();
-  a.cand_inst = s->cinst.as<uint32_t>();
+  a.cand_blk = s->cinst.as<uint32_t>();
   a.cand_cap = cand_cap;
   a.deferred[0] = s->def0.as<uint64_t>();
   a.deferred[1] = s->def1.as<uint64_t>();
@@ -182,17 +189,25 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   a.stamps = o.por_jobs ? 1u : 0u;
   a.por_threads = 256;
   a.por_blocks = static_cast<uint32_t>(prop.multiProcessorCount) * 4;
-  // capture rounds_per_step rounds once; replays reuse the fixed pools
+  a.chunk_states = o.threads;  // 64 states per wave
+  a.expand_lds = static_cast<uint32_t>(batch_expand_lds(o.threads, o.max_procs, rows16));
+  // capture rounds_per_step rounds once per slot format; replays reuse the
+  // fixed pools
   SV_TRY(hipStreamSynchronize(s->st));
-  SV_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-  hipError_t ce = hipSuccess;
-  for (uint32_t r = 0; r < o.rounds_per_step && ce == hipSuccess; r++)
-    ce = launch_round_w(a, s->W, s->BW, s->o.blocks, s->o.threads,
-                        static_cast<uint32_t>(prop.multiProcessorCount) * 8, s->st);
-  hipError_t ee = hipStreamEndCapture(s->st, &s->graph);
-  SV_TRY(ce);
-  SV_TRY(ee);
-  SV_TRY(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+  for (int stamped = 0; stamped < (o.por_jobs ? 2 : 1); stamped++) {
+    BatchArgs ca = a;
+    ca.stamps = static_cast<uint32_t>(stamped);
+    SV_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    for (uint32_t r = 0; r < o.rounds_per_step && ce == hipSuccess; r++)
+      ce = launch_round_w(ca, s->W, s->BW, s->o.blocks, s->o.threads, s->o.insert_blocks, s->st);
+    hipGraph_t &g = stamped ? s->graph : s->graph_nostamp;
+    hipGraphExec_t &x = stamped ? s->exec : s->exec_nostamp;
+    hipError_t ee = hipStreamEndCapture(s->st, &g);
+    SV_TRY(ce);
+    SV_TRY(ee);
+    SV_TRY(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+  }
   SV_TRY(hipStreamSynchronize(s->st));
   s->running.resize(S);
   s->slot_used.assign(S, 0);
@@ -331,11 +346,15 @@ extern "C" int gx_server_step(gx_server *s, gx_server_status *status) {
     SV_TRY(hipMemcpyAsync(&s->a.ctrl->prefix_only, &s->one, 4, hipMemcpyHostToDevice, s->st));
     SV_TRY(launch_batch_advance(s->a, s->st));
   }
-  // 2. one replay of the captured rounds
-  bool any = false;
-  for (uint32_t i = 0; i < s->o.slots; i++) any = any || s->slot_used[i];
+  // 2. one replay of the captured rounds (with level stamps while a job with
+  // the cycle proviso runs)
+  bool any = false, stamps = false;
+  for (uint32_t i = 0; i < s->o.slots; i++) {
+    any = any || s->slot_used[i];
+    stamps = stamps || (s->slot_used[i] && s->running[i].mode == kModePorProviso);
+  }
   if (any) {
-    SV_TRY(hipGraphLaunch(s->exec, s->st));
+    SV_TRY(hipGraphLaunch(stamps ? s->exec : s->exec_nostamp, s->st));
     s->steps++;
   }
   // 3. retire finished jobs

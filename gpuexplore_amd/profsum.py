@@ -1,17 +1,31 @@
 """rocprofv3 collection and the summaries kept under profiles/.
 
-Collect three separate runs of one command (kernel trace with statistics,
-then the FETCH_SIZE counter, then WRITE_SIZE; counters are never combined
-with tracing), then summarise them:
+Collects separate runs of one command -- a kernel trace with statistics,
+then two counter passes (counters are never combined with tracing, and each
+pass stays within the four TCC counters one run can hold) -- and summarises
+them:
 
     python -m gpuexplore_amd.profsum collect --out /abs/dir -- \\
         python3 $PWD/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0
     python -m gpuexplore_amd.profsum summarize --dir /abs/dir --name k_insert_config2
 
-The counters are scaled by what they read for bench.py's streaming copy
-(k_copy, 4 GiB read and 4 GiB written per dispatch) in the same process.
-Algorithmic bytes per FIND-OR-PUT launch come from the bench line the
-kernel-trace run printed (128 B x (lookups + inserts) per launch).
+Bytes moved between the L2 (TCC) and memory are counted from the L2's
+memory-side requests by size, with no scale factor:
+
+    read  = 32 x TCC_EA0_RDREQ_32B + 64 x TCC_EA0_RDREQ_64B + 128 x TCC_EA0_RDREQ_128B
+    write = 64 x TCC_EA0_WRREQ_64B + 32 x (TCC_EA0_WRREQ - TCC_EA0_WRREQ_64B)
+
+The same process runs kernels of known bytes per dispatch (bench.py's
+streaming copy and read, 4 GiB each, and 2^29 random 64-byte bucket reads);
+their readings are reported beside the known bytes, which validates the
+accounting.  (The derived FETCH_SIZE counts a 128-byte request as 64 bytes
+unless the L2 also counts it in TCC_BUBBLE, so it read half of a streaming
+copy's reads.)  Requests to memory may be served by the MALL (Infinity
+Cache) or by HBM; the L2 cannot tell them apart.
+
+TCC_ATOMIC counts every atomic request the L2 serves (CAS claims, frontier
+and counter additions).  Algorithmic bytes per FIND-OR-PUT launch come from
+the bench line the kernel-trace run printed (128 B x (lookups + inserts)).
 """
 
 from __future__ import annotations
@@ -27,9 +41,12 @@ from collections import defaultdict
 from typing import Dict, List
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-COPY_BYTES = 4 << 30  # per k_copy dispatch, read and written (bench.py)
-PASSES = [("kt", ["--kernel-trace", "--stats"]), ("fetch", ["--pmc", "FETCH_SIZE"]),
-          ("write", ["--pmc", "WRITE_SIZE"])]
+RD = ["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"]
+WR = ["TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum", "TCC_ATOMIC_sum", "TCC_BUBBLE_sum"]
+PASSES = [("kt", ["--kernel-trace", "--stats"]), ("rd", ["--pmc", *RD]), ("wr", ["--pmc", *WR])]
+# known bytes per dispatch of bench.py's measurement kernels: (read, written)
+KNOWN = {"k_copy": (4 << 30, 4 << 30), "k_read": (4 << 30, 0), "k_random_read": (64 << 29, 0)}
+BENCH_CMD = "python3 $PWD/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0"
 
 
 def collect(out: str, cmd: List[str], limit: int) -> int:
@@ -46,6 +63,7 @@ def collect(out: str, cmd: List[str], limit: int) -> int:
         if rc != 0 and not (rc in (139, -11) and done):  # the profiler may segfault after a complete run
             print(f"pass {name} failed with {rc}", file=sys.stderr)
             return rc or 1
+        print(f"profile pass {name} done", flush=True)
     return 0
 
 
@@ -57,17 +75,38 @@ def _one(pattern: str) -> str:
 
 
 def _short(name: str) -> str:
-    return name.split("(")[0].strip()
+    """kernel name without its argument list and return type"""
+    n = name.split("(")[0].strip()
+    return n[5:] if n.startswith("void ") else n
+
+
+def _base(name: str) -> str:
+    """kernel name without namespace and template arguments"""
+    return _short(name).split("<")[0].split("::")[-1]
 
 
 def _counters(path: str):
-    per = defaultdict(float)
-    calls = defaultdict(int)
+    """per kernel: {counter: total}, and dispatches per kernel"""
+    per = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
     for r in csv.DictReader(open(path)):
         k = _short(r["Kernel_Name"])
-        per[k] += float(r["Counter_Value"])
-        calls[k] += 1
-    return per, calls
+        per[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[k].add(r.get("Dispatch_Id", r.get("Correlation_Id", "")))
+    return per, {k: len(v) for k, v in disp.items()}
+
+
+def traffic(rd: Dict[str, float], wr: Dict[str, float]) -> Dict[str, float]:
+    read = 32 * rd["TCC_EA0_RDREQ_32B_sum"] + 64 * rd["TCC_EA0_RDREQ_64B_sum"] + 128 * rd["TCC_EA0_RDREQ_128B_sum"]
+    write = 64 * wr["TCC_EA0_WRREQ_64B_sum"] + 32 * (wr["TCC_EA0_WRREQ_sum"] - wr["TCC_EA0_WRREQ_64B_sum"])
+    unsized = rd["TCC_EA0_RDREQ_sum"] - (rd["TCC_EA0_RDREQ_32B_sum"] + rd["TCC_EA0_RDREQ_64B_sum"] +
+                                         rd["TCC_EA0_RDREQ_128B_sum"])
+    fetch_size = (wr["TCC_BUBBLE_sum"] * 128 + 32 * rd["TCC_EA0_RDREQ_32B_sum"] +
+                  64 * (rd["TCC_EA0_RDREQ_sum"] - wr["TCC_BUBBLE_sum"] - rd["TCC_EA0_RDREQ_32B_sum"]))
+    return {"read_bytes": read, "write_bytes": write, "atomics": wr["TCC_ATOMIC_sum"],
+            "read_requests": rd["TCC_EA0_RDREQ_sum"], "read_requests_unsized": unsized,
+            "req_32b": rd["TCC_EA0_RDREQ_32B_sum"], "req_64b": rd["TCC_EA0_RDREQ_64B_sum"],
+            "req_128b": rd["TCC_EA0_RDREQ_128B_sum"], "fetch_size_expr_bytes": fetch_size}
 
 
 def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
@@ -79,56 +118,87 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
         raise RuntimeError("no bench line in the kernel-trace run")
     stats = list(csv.DictReader(open(_one(os.path.join(d, "kt", "**", "*kernel_stats.csv")))))
     trace = list(csv.DictReader(open(_one(os.path.join(d, "kt", "**", "*kernel_trace.csv")))))
-    level = [r for r in trace if kernel in r["Kernel_Name"] and int(r.get("Grid_Size", r.get("Grid_Size_X", "0")) or 0) > 64]
+
+    def grid(r):
+        return int(r.get("Grid_Size", r.get("Grid_Size_X", "0")) or 0)
+
+    level = [r for r in trace if kernel in r["Kernel_Name"] and grid(r) > 64]
     durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in level]
-    fetch, fcalls = _counters(_one(os.path.join(d, "fetch", "**", "*counter_collection.csv")))
-    write, wcalls = _counters(_one(os.path.join(d, "write", "**", "*counter_collection.csv")))
-    copy = next(k for k in fetch if "k_copy" in k)
-    fscale = COPY_BYTES / (fetch[copy] / fcalls[copy])
-    wscale = COPY_BYTES / (write[copy] / wcalls[copy])
-    kern = {}
-    for k in fetch:
+    rd, rdisp = _counters(_one(os.path.join(d, "rd", "**", "*counter_collection.csv")))
+    wr, wdisp = _counters(_one(os.path.join(d, "wr", "**", "*counter_collection.csv")))
+    kern, check = {}, {}
+    for k in rd:
+        if k not in wr:
+            continue
+        t = traffic(rd[k], wr[k])
+        n = rdisp[k]
+        base = _base(k)
+        if base in KNOWN:
+            kr, kw = KNOWN[base]
+            check[k] = {"dispatches": n, "known_read_bytes": kr, "known_write_bytes": kw,
+                        "read_bytes": t["read_bytes"] / n, "write_bytes": t["write_bytes"] / n,
+                        "read_ratio": t["read_bytes"] / n / kr,
+                        "write_ratio": (t["write_bytes"] / n / kw) if kw else None,
+                        "fetch_size_expr_ratio": t["fetch_size_expr_bytes"] / n / kr,
+                        "req_64b_per_dispatch": t["req_64b"] / n, "req_128b_per_dispatch": t["req_128b"] / n}
         if "k_expand" in k or "k_insert" in k or "k_fused" in k:
-            n_level = sum(1 for r in trace if _short(r["Kernel_Name"]) == k and
-                          int(r.get("Grid_Size", r.get("Grid_Size_X", "0")) or 0) > 64)
-            n_level = n_level or fcalls[k]
-            kern[k] = {"fetch_bytes": fetch[k] * fscale, "write_bytes": write.get(k, 0.0) * wscale,
-                       "dispatches": fcalls[k], "level_launches": n_level,
-                       "bytes_per_launch": (fetch[k] * fscale + write.get(k, 0.0) * wscale) / n_level}
+            n_level = sum(1 for r in trace if _short(r["Kernel_Name"]) == k and grid(r) > 64) or n
+            kern[k] = {"dispatches": n, "level_launches": n_level,
+                       "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"],
+                       "bytes_per_launch": (t["read_bytes"] + t["write_bytes"]) / n_level,
+                       "dram_bytes_per_launch": (t["read_bytes"] + t["write_bytes"]) / n_level,
+                       "atomics_per_launch": t["atomics"] / n_level,
+                       "read_requests_per_launch": t["read_requests"] / n_level,
+                       "req_32b": t["req_32b"], "req_64b": t["req_64b"], "req_128b": t["req_128b"],
+                       "read_requests_unsized": t["read_requests_unsized"]}
     roof = bench["roofline"]
-    cmd = "python3 $PWD/bench.py --steps 1 --warmup 0 --no-cpu --no-batch --no-por --serve-seconds 0"
-    data = {"workload": bench["config"]["workload"], "command": f"rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -- {cmd}",
-            "fetch_scale": fscale, "write_scale": wscale, "kernels": kern,
+    avg_us = sum(durs) / len(durs) / 1e3 if durs else None
+    data = {"workload": bench["config"]["workload"],
+            "command": f"rocprofv3 --pmc {' '.join(RD)} | --pmc {' '.join(WR)} -- {BENCH_CMD}",
+            "accounting": "read = 32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B; "
+                          "write = 64 WRREQ_64B + 32 (WRREQ - WRREQ_64B); no scale factor",
+            "validation": check, "kernels": kern,
             "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
-            "trace_avg_launch_us": sum(durs) / len(durs) / 1e3 if durs else None,
-            "bench_avg_launch_us": roof["avg_launch_us"]}
+            "trace_avg_launch_us": avg_us, "bench_avg_launch_us": roof["avg_launch_us"]}
     os.makedirs(out_dir, exist_ok=True)
     with open(os.path.join(out_dir, f"{name}_traffic.json"), "w") as f:
         json.dump(data, f, indent=1)
     md = [f"# {kernel} on {bench['config']['workload']}", "",
-          "Kernel time, `rocprofv3 --kernel-trace --stats`, one exploration:", "",
           "From the repository root (each profiler pass runs in /tmp with TMPDIR=/tmp; <dir> absolute):", "",
-          "```", f"python3 -m gpuexplore_amd.profsum collect --out <dir> -- {cmd}",
+          "```", f"python3 -m gpuexplore_amd.profsum collect --out <dir> -- {BENCH_CMD}",
           f"python3 -m gpuexplore_amd.profsum summarize --dir <dir> --name {name}", "```", "",
+          "Kernel time, `rocprofv3 --kernel-trace --stats`, one exploration:", "",
           "| kernel | calls | total ms | average us |", "|---|---|---|---|"]
-    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:8]:
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:10]:
         md.append(f"| `{r['Name'][:60]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
                   f"{float(r['AverageNs']) / 1e3:.1f} |")
     md += ["", f"{kernel} average over its {len(durs)} level launches (the 1-candidate initial launch excluded): "
-               f"{data['trace_avg_launch_us']:.0f} us (kernel trace); bench.py's HIP-event average for the same "
-               f"launches: {roof['avg_launch_us']:.0f} us.", "",
-           "HBM traffic, `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs of the same command, scaled by "
-           "the counters' reading of bench.py's 4 GiB streaming copy (k_copy) in the same process:", "",
-           f"- counters read KiB: FETCH_SIZE x {fscale / 1024:.3f} KiB, WRITE_SIZE x {wscale / 1024:.3f} KiB per "
-           "count gives bytes (FETCH_SIZE reads half of what the copy fetches).",
-           f"- measured streaming copy (bench.py): {bench['roofline'].get('measured_copy_peak_gbps', 0):.0f} GB/s.", "",
-           "| kernel | level launches | fetch GB | write GB | GB per level launch |", "|---|---|---|---|---|"]
+               f"{avg_us:.0f} us (kernel trace); bench.py's HIP-event average for the same launches: "
+               f"{roof['avg_launch_us']:.0f} us.", "",
+           "Bytes between L2 and memory, from two counter runs of the same command "
+           f"(`--pmc {' '.join(RD)}`, `--pmc {' '.join(WR)}`): read = 32 x RDREQ_32B + 64 x RDREQ_64B + "
+           "128 x RDREQ_128B, write = 64 x WRREQ_64B + 32 x the other write requests, no scale factor.", "",
+           "Validation on kernels of known bytes in the same process:", "",
+           "| kernel | dispatches | known read GB | counted read GB | ratio | known write GB | counted write GB | "
+           "FETCH_SIZE expression / known read |", "|---|---|---|---|---|---|---|---|"]
+    for k, v in sorted(check.items()):
+        md.append(f"| `{k}` | {v['dispatches']} | {v['known_read_bytes'] / 1e9:.2f} | {v['read_bytes'] / 1e9:.2f} | "
+                  f"{v['read_ratio']:.3f} | {v['known_write_bytes'] / 1e9:.2f} | {v['write_bytes'] / 1e9:.2f} | "
+                  f"{v['fetch_size_expr_ratio']:.3f} |")
+    md += ["", "| kernel | level launches | read GB | write GB | GB per launch | 64 B / 128 B read requests | "
+               "atomics per launch |", "|---|---|---|---|---|---|---|"]
     for k, v in kern.items():
-        md.append(f"| `{k}` | {v['level_launches']} | {v['fetch_bytes'] / 1e9:.1f} | {v['write_bytes'] / 1e9:.1f} | "
-                  f"{v['bytes_per_launch'] / 1e9:.2f} |")
-    md += ["", f"Algorithmic bytes per {kernel} launch (128 B x (lookups + inserts) per launch): "
-               f"{roof['algorithmic_bytes_per_launch'] / 1e9:.2f} GB; achieved {roof['achieved']:.0f} GB/s = "
-               f"{100 * roof['frac']:.1f} % of {roof['peak']:.0f} GB/s.", ""]
+        md.append(f"| `{k}` | {v['level_launches']} | {v['read_bytes'] / 1e9:.1f} | {v['write_bytes'] / 1e9:.1f} | "
+                  f"{v['bytes_per_launch'] / 1e9:.2f} | {v['req_64b'] / 1e9:.2f} G / {v['req_128b'] / 1e9:.2f} G | "
+                  f"{v['atomics_per_launch'] / 1e6:.1f} M |")
+    ki = next((v for k, v in kern.items() if kernel in k), None)
+    if ki and avg_us:
+        phys = ki["bytes_per_launch"] / (avg_us * 1e-6) / 1e9
+        md += ["", f"Algorithmic bytes per {kernel} launch (128 B x (lookups + inserts)): "
+                   f"{roof['algorithmic_bytes_per_launch'] / 1e9:.2f} GB, {roof['achieved']:.0f} GB/s = "
+                   f"{100 * roof['frac']:.1f} % of {roof['peak']:.0f} GB/s.  Counted bytes to and from memory: "
+                   f"{ki['bytes_per_launch'] / 1e9:.2f} GB per launch, {phys:.0f} GB/s = {100 * phys / roof['peak']:.1f} % "
+                   f"of the peak; atomics {ki['atomics_per_launch'] / (avg_us * 1e-6) / 1e9:.2f} G/s.", ""]
     with open(os.path.join(out_dir, f"{name}.md"), "w") as f:
         f.write("\n".join(md))
     return data

@@ -114,9 +114,9 @@ def plain_path(jobs: List[dict]) -> List[dict]:
         t = time.monotonic()
         while True:
             r = gx.explore(m, property=j["prop"], por=int(j["por"]), q_ints=q)
-            if not r["table_full"] or q >= (1 << 32):
+            if not r["table_full"] or q >= (1 << 34):
                 break
-            q *= 4
+            q = min(q * 4, 1 << 34)
         r["wall_s"] = time.monotonic() - t
         out.append(r)
     return out

@@ -224,6 +224,8 @@ typedef struct gx_server_options {
   uint64_t hash_seed;
   uint32_t model_words;      /* largest compiled model accepted (32-bit words) */
   uint32_t por_jobs;         /* 1: POR jobs accepted (each round also runs the POR kernel) */
+  uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per round (0 = auto) */
+  uint32_t reserved;
 } gx_server_options;
 
 typedef struct gx_server_status {

@@ -178,8 +178,8 @@ class OracleModel:
         return lib().orc_por_select(self.ptr, self._vec(words), method)
 
     def explore(self, prop: int = PROP_NONE, por: bool = False, proviso: bool = False,
-                max_states: int = 0, seed: int = 0, method: int = POR_CAMPLE) -> dict:
-        o = _Opts(prop, int(por), int(proviso), max_states, seed, method)
+                max_states: int = 0, seed: int = 0, method: int = POR_CAMPLE, expected_states: int = 0) -> dict:
+        o = _Opts(prop, int(por), int(proviso), max_states, seed, method, expected_states)
         r = _Res()
         rc = lib().orc_explore(self.ptr, ctypes.byref(o), ctypes.byref(r))
         if rc != 0:
@@ -188,9 +188,11 @@ class OracleModel:
                     full=r.full_successors, violation=bool(r.violation),
                     truncated=bool(r.truncated), seconds=r.seconds)
 
-    def explore_mt(self, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:
-        """orc_explore_mt: the plain method on `threads` threads."""
-        o = _Opts(prop, 0, 0, max_states, seed, POR_CAMPLE)
+    def explore_mt(self, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0,
+                   expected_states: int = 0) -> dict:
+        """orc_explore_mt: the plain method on `threads` threads (the visited
+        set presized for `expected_states` when given)."""
+        o = _Opts(prop, 0, 0, max_states, seed, POR_CAMPLE, expected_states)
         r = _Res()
         rc = lib().orc_explore_mt(self.ptr, ctypes.byref(o), threads, ctypes.byref(r))
         if rc != 0:
