@@ -99,6 +99,7 @@ class BatchStats(ctypes.Structure):
         ("lookups", ctypes.c_uint64),
         ("states", ctypes.c_uint64),
         ("transitions", ctypes.c_uint64),
+        ("cand_entries", ctypes.c_uint64),
     ]
 
 
@@ -117,7 +118,8 @@ class ServerOptions(ctypes.Structure):
         ("model_words", ctypes.c_uint32),
         ("por_jobs", ctypes.c_uint32),
         ("insert_blocks", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("large_slots", ctypes.c_uint32),
+        ("large_states", ctypes.c_uint64),
     ]
 
 

@@ -48,6 +48,7 @@ struct BatchCtrl {
   uint32_t cand_overflow;  // some instance stopped with a chunk larger than the candidate buffer
   uint32_t prefix_only;  // next advance only recomputes the chunk prefix
   uint32_t pad;
+  uint64_t cand_entries;  // candidate-buffer entries of all rounds (candidates + released gaps)
 };
 
 struct BatchArgs {
@@ -80,16 +81,17 @@ inline uint32_t batch_mode(int por, int cycle_proviso) {
 // unfinished instance (one BFS level)
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
                           uint32_t ins_blocks, hipStream_t s);
-// LDS of the plain batch expand kernel: the waves' row tables for the
-// largest model (at most 64 KiB) plus room to stage one model section of up
-// to kBatchModelLds bytes beside them, capped at 64 KiB
-constexpr size_t kBatchModelLds = 30 * 1024;
+// LDS of the plain batch expand kernel: 40 KiB (four workgroups per CU, as
+// the single-model kernel gets on a 20-process model), or the waves' row
+// tables for the largest model when they need more.  A model is staged when
+// its section and its own row tables fit.
+constexpr size_t kBatchExpandLds = 40 * 1024;
 inline size_t batch_rows_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
   return static_cast<size_t>(threads / 64) * max_procs * 64 * (rows16 ? 2 : 4);
 }
 inline size_t batch_expand_lds(uint32_t threads, uint32_t max_procs, bool rows16) {
-  const size_t b = batch_rows_lds(threads, max_procs, rows16) + kBatchModelLds;
-  return b < 64 * 1024 ? b : 64 * 1024;
+  const size_t r = batch_rows_lds(threads, max_procs, rows16);
+  return r > kBatchExpandLds ? r : kBatchExpandLds;
 }
 // the advance step alone (with prefix_only set: chunk prefix after admissions)
 hipError_t launch_batch_advance(const BatchArgs &a, hipStream_t s);

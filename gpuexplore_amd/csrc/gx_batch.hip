@@ -221,13 +221,18 @@ __global__ __launch_bounds__(512) void k_batch_expand(BatchArgs a) {
         __syncthreads();
         staged = layout;
       }
-      const ModelHdr H = lds ? tps_rebased(Hg) : Hg;
-      const uint32_t *M = lds ? smem : Mg;
-      RowT *rows = lds ? reinterpret_cast<RowT *>(smem + sec_w) + wib * (Hg.nprocs * 64u)
-                       : reinterpret_cast<RowT *>(smem) + wib * (a.max_procs * 64u);
+      // two call sites, so that each knows its model's address space (a
+      // pointer that may be either would make every model read a FLAT access)
       const uint64_t start = (c - I->chunk_prefix) * a.chunk_states + wib * 64u;
-      batch_chunk<W, RowT>(a, C, ch, acc, i, start, M, H, rows, (nchunks - c + gridDim.x - 1) / gridDim.x,
-                           def_out, def_out_count);
+      const uint64_t left = (nchunks - c + gridDim.x - 1) / gridDim.x;
+      if (lds)
+        batch_chunk<W, RowT>(a, C, ch, acc, i, start, smem, tps_rebased(Hg),
+                             reinterpret_cast<RowT *>(smem + sec_w) + wib * (Hg.nprocs * 64u), left, def_out,
+                             def_out_count);
+      else
+        batch_chunk<W, RowT>(a, C, ch, acc, i, start, Mg, Hg,
+                             reinterpret_cast<RowT *>(smem) + wib * (a.max_procs * 64u), left, def_out,
+                             def_out_count);
     }
   }
   acc.flush(a.inst);
@@ -372,7 +377,7 @@ constexpr int kBatchStage = 256;
 // bits and level) and hashes it once, and the step's iterations hand all of
 // that to the lane groups in candidate order.
 template <int W, int BW, bool STAMPS>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<W, STAMPS>)))
+__global__ __launch_bounds__(512)
 void k_batch_insert(BatchArgs a) {
   extern __shared__ uint64_t bstage_all[];
   BatchCtrl *C = a.ctrl;
@@ -413,77 +418,69 @@ void k_batch_insert(BatchArgs a) {
     wave_sync();
     staged = 0;
   };
-  // What a candidate's lookup needs besides its key: its instance, the
-  // partition, its first bucket and hash, and (STAMPS) the slot format.
-  struct Cand {
-    uint32_t i, K, stamp;
-    uint64_t tb, nb, h, b0;
+  // A step of 64 candidates (or an iteration of CPW) lies in one 64-entry
+  // block of the candidate buffer, so its instance is wave-uniform: one
+  // scalar read of the block's id, and the partition (first bucket, size,
+  // and with stamps the slot format's state bits and level) is read again
+  // only when the instance changes.
+  uint32_t pi = kNoInst;
+  uint64_t ptb = 0, pnb = 1;
+  uint32_t pK = 63u * W, pstamp = 0;
+  const uint32_t ninst = C->n;
+  // the id of the block holding candidate c (a wave-uniform read)
+  auto block_id = [&](uint64_t c) {
+    const uint64_t blk = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(c >> 38))) << 32) |
+                         __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(c >> 6));
+    return a.cand_blk[blk];
   };
-  // The instance's partition, read once for the whole wave when every
-  // active lane's candidate belongs to one instance (long runs of one
-  // instance are the rule), else per lane.
-  auto part_of = [&](Cand &r, const BatchInst *I, bool active) {
-    r.tb = active ? I->tb : 0;
-    r.nb = active ? I->nb : 1;
-    if (STAMPS) {
-      r.K = (active && I->mode == kModePorProviso) ? I->hdr.state_bits : 63u * W;
-      r.stamp = active ? I->level + 1u : 0u;  // new states are stamped with their level
+  auto partition = [&](uint32_t id) {
+    uint32_t i = __builtin_amdgcn_readfirstlane(id);
+    i = i < ninst ? i : 0u;  // a block with an active entry holds a valid id; stay in bounds anyway
+    if (i != pi) {
+      const BatchInst *I = &a.inst[i];
+      pi = i;
+      ptb = I->tb;
+      pnb = I->nb;
+      if (STAMPS) {
+        pK = I->mode == kModePorProviso ? I->hdr.state_bits : 63u * W;
+        pstamp = I->level + 1u;  // new states are stamped with the level they are found at
+      }
     }
   };
-  auto resolve = [&](const uint64_t (&key)[W], uint64_t ci, bool active) {
-    Cand r{};
-    r.i = active ? a.cand_blk[ci >> 6] : kNoInst;
-    const uint64_t am = ballot(active);
-    const uint32_t i0 = am ? static_cast<uint32_t>(__shfl(static_cast<int>(r.i), __ffsll(static_cast<unsigned long long>(am)) - 1, 64))
-                           : 0u;
-    if (!ballot(active && r.i != i0)) {
-      part_of(r, &a.inst[__builtin_amdgcn_readfirstlane(i0)], active);
-    } else {
-      part_of(r, &a.inst[active ? r.i : 0u], active);
-    }
-    if (!active) r.i = 0;
-    r.h = key_hash<W>(key, a.seed);
-    r.b0 = bucket_of(r.h, 0, r.nb);
-    return r;
-  };
-  auto visit = [&](const uint64_t (&key)[W], const Cand &c, bool active) {
-    const Table T{a.table + c.tb * BW, c.nb, a.seed};
+  auto visit = [&](const uint64_t (&key)[W], uint64_t h, uint64_t b0, bool active) {
+    const Table T{a.table + ptb * BW, pnb, a.seed};
     SlotFmt<W> f;
     if (STAMPS)
-      make_fmt<W, true>(f, c.K, c.stamp);
+      make_fmt<W, true>(f, pK, pstamp);
     else
       make_fmt<W, false>(f, 0, 0);
     uint64_t v[W];
     slot_value<W, W>(key, f, v);
-    const uint64_t h = c.h;
-    const int st = find_or_put_group_b<W, BW>(T, v, f, c.b0, [h]() { return h; }, sub, gbase, active);
+    const int st = find_or_put_group_b<W, BW>(T, v, f, b0, [h]() { return h; }, sub, gbase, active);
     acc_look += __popcll(ballot(active && sub == 0));
     if (active && sub == 0) {
-      if (st == kFull) a.inst[c.i].table_full = 1;
-      if (st == kTimeout) a.inst[c.i].spin_timeout = 1;
+      if (st == kFull) a.inst[pi].table_full = 1;
+      if (st == kTimeout) a.inst[pi].spin_timeout = 1;
     }
     const bool is_new = active && st == kInserted;
-    uint64_t m = ballot(is_new && sub == 0);
-    while (m) {  // one group per instance present in this iteration
-      const uint32_t i0 = __shfl(c.i, __ffsll(static_cast<unsigned long long>(m)) - 1, 64);
-      const uint64_t mm = ballot(is_new && sub == 0 && c.i == i0);
-      const uint32_t cnt = __popcll(mm);
-      if (staged && (stage_inst != i0 || staged + cnt > static_cast<uint32_t>(kBatchStage))) flush();
-      stage_inst = i0;
-      if (is_new && c.i == i0) {
-        const uint32_t pos = staged + __popcll(mm & ((1ull << gbase) - 1ull));
+    const uint64_t m = ballot(is_new && sub == 0);
+    if (!m) return;
+    const uint32_t cnt = __popcll(m);
+    if (staged && (stage_inst != pi || staged + cnt > static_cast<uint32_t>(kBatchStage))) flush();
+    stage_inst = pi;
+    if (is_new) {
+      const uint32_t pos = staged + __popcll(m & ((1ull << gbase) - 1ull));
 #pragma unroll
-        for (int j = 0; j < W; j++)
-          if (j % LPB == sub) stage[pos * W + j] = key[j];
-      }
-      staged += cnt;
-      m &= ~mm;
+      for (int j = 0; j < W; j++)
+        if (j % LPB == sub) stage[pos * W + j] = key[j];
     }
+    staged += cnt;
   };
   const uint64_t iters = insert_run(n, nw * CPW);
   if (iters < kInsertSpread) {
     const uint64_t RUN = CPW * iters;
     for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
+      const uint64_t cstep = c0;
       const uint64_t ci = c0 + cg;
       c0 += CPW;
       if (c0 == r0 + RUN) {
@@ -494,41 +491,51 @@ void k_batch_insert(BatchArgs a) {
 #pragma unroll
       for (int j = 0; j < W; j++) key[j] = ci < n ? __builtin_nontemporal_load(a.cand + ci * W + j) : 0;
       const bool active = key[0] != 0;  // past the end or a released reservation: zero
-      visit(key, resolve(key, ci, active), active);
+      if (!ballot(active)) continue;
+      partition(block_id(cstep));
+      const uint64_t h = key_hash<W>(key, a.seed);
+      visit(key, h, bucket_of(h, 0, pnb), active);
     }
   } else {
     static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
     const uint64_t RUN = CPW * kInsertRun;
-    for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
-      const uint64_t cl = c0 + lane;
+    // Software-pipelined by one step: the next step's keys and block id are
+    // read while this step's lookups run (the block id decides the partition,
+    // hence the first bucket, so reading it in the step itself would put one
+    // more memory round trip before every step's lookups).
+    uint64_t r0 = gw * RUN, c0 = r0;
+    uint64_t key_n[W];
+    uint32_t id_n = 0;
+    auto fetch = [&](uint64_t cs) {
+      const uint64_t cl = cs + lane;
+#pragma unroll
+      for (int j = 0; j < W; j++) key_n[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
+      id_n = block_id(cs);
+    };
+    if (c0 < n) fetch(c0);
+    while (c0 < n) {
+      uint64_t key_l[W];
+#pragma unroll
+      for (int j = 0; j < W; j++) key_l[j] = key_n[j];
+      const uint32_t id = id_n;
       c0 += 64;
       if (c0 == r0 + RUN) {
         r0 += nw * RUN;
         c0 = r0;
       }
-      uint64_t key_l[W];
-#pragma unroll
-      for (int j = 0; j < W; j++) key_l[j] = cl < n ? __builtin_nontemporal_load(a.cand + cl * W + j) : 0;
-      const bool act_l = key_l[0] != 0;
-      const uint64_t act_m = ballot(act_l);
-      const Cand c_l = resolve(key_l, cl, act_l);
+      if (c0 < n) fetch(c0);
+      const uint64_t act_m = ballot(key_l[0] != 0);
+      if (!act_m) continue;
+      partition(id);
+      const uint64_t h_l = key_hash<W>(key_l, a.seed);
+      const uint64_t b_l = bucket_of(h_l, 0, pnb);
 #pragma unroll 1
       for (int s2 = 0; s2 < LPB; s2++) {
         const int src = s2 * CPW + cg;
         uint64_t key[W];
 #pragma unroll
         for (int j = 0; j < W; j++) key[j] = __shfl(key_l[j], src, 64);
-        Cand c;
-        c.i = __shfl(c_l.i, src, 64);
-        c.tb = __shfl(c_l.tb, src, 64);
-        c.nb = __shfl(c_l.nb, src, 64);
-        c.h = __shfl(c_l.h, src, 64);
-        c.b0 = __shfl(c_l.b0, src, 64);
-        if (STAMPS) {
-          c.K = __shfl(c_l.K, src, 64);
-          c.stamp = __shfl(c_l.stamp, src, 64);
-        }
-        visit(key, c, (act_m >> src) & 1ull);
+        visit(key, __shfl(h_l, src, 64), __shfl(b_l, src, 64), (act_m >> src) & 1ull);
       }
     }
   }
@@ -550,6 +557,7 @@ __global__ __launch_bounds__(1024) void k_batch_advance(BatchArgs a) {
   const bool prefix_only = C->prefix_only != 0;  // recompute chunks after admissions
   __syncthreads();
   if (tid == 0) {
+    C->cand_entries += C->cand_count < a.cand_cap ? C->cand_count : a.cand_cap;
     C->cand_count = 0;
     if (!prefix_only) C->rounds += 1;
   }
@@ -976,6 +984,7 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
     stats->rounds = ch.rounds;
     stats->graph_launches = launches;
     stats->lookups = ch.lookups;
+    stats->cand_entries = ch.cand_entries;
     for (int i = 0; i < n; i++) {
       stats->states += inst[i].inserts;
       stats->transitions += inst[i].transitions;

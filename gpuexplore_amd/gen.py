@@ -448,12 +448,12 @@ def config5_job(seed: int) -> Tuple[Network, str, int, bool]:
     switches it off (cu:1851-1874) -- and 10% from the config-2 generator at
     full scale (20 processes, 16 local states, token budgets, -d).  Returns
     (network, flag, estimated states, por).  The POR jobs' reduced spaces
-    hold 7e4..4e6 states (mean about 1e6, 40 seeds), the config-2 jobs about
-    3e8; the estimates are what a client would guess, and the server
+    hold 7e4..4e6 states (mean about 1e6, 40 seeds), the config-2 jobs
+    3e8..6e8; the estimates are what a client would guess, and the server
     restarts a job whose partition fills."""
     rng = Rng(0xC5 * 1000003 + seed)
     if rng.below(10) == 0:
-        return config2(seed), "-d", 350_000_000, False
+        return config2(seed), "-d", 500_000_000, False
     return rounds(24, 8, 4, [1] * 24, 4, seed, mix=(0.7, 0.2, 0.1), phased=True), "-d", 2_000_000, True
 
 

@@ -55,10 +55,10 @@ def pct(xs: List[float], q: float) -> float:
     return s[min(len(s) - 1, max(0, int(math.ceil(q * len(s))) - 1))]
 
 
-# Server pools for the stream: a 48 GiB visited set (frontier pools of the
-# same size), enough for four of the 3e8-state jobs at their estimate beside
-# the POR jobs.
-SERVER_KW = {"table_bytes": 48 << 30, "page_bytes": 4 << 20}
+# Server pools for the stream: a 96 GiB visited set (frontier pools of a
+# quarter of its slots each), enough for six of the 5e8-state jobs at their
+# estimate beside the POR jobs.
+SERVER_KW = {"table_bytes": 96 << 30, "page_bytes": 4 << 20}
 
 
 def run_stream(jobs: List[dict], times: Optional[List[float]], server_kw: Optional[dict] = None,
@@ -102,18 +102,29 @@ def run_stream(jobs: List[dict], times: Optional[List[float]], server_kw: Option
             "mean_s": sum(lat) / n, "restarts": sum(res[i]["restarts"] for i in range(n))}
 
 
-def plain_path(jobs: List[dict]) -> List[dict]:
+def plain_path(jobs: List[dict], log=None) -> List[dict]:
     """Each job alone through gx_explore: the reference for every verdict and
     the service time of a one-at-a-time server.  The table is sized from the
     estimate like the server's partition and grown 4x after a full table; the
     service time includes every attempt."""
     out = []
-    for j in jobs:
+    t_log = time.monotonic()
+    for k, j in enumerate(jobs):
+        if log and time.monotonic() - t_log > 10:
+            t_log = time.monotonic()
+            log(f"plain path: {k} of {len(jobs)} jobs done")
         m = gx.Model.from_text(j["gpf"], por=j["por"])
         q = 1 << max(20, (8 * j["est"]).bit_length())  # 32-bit ints: >= 4 per estimated state
+        # buffers sized from the estimate, not from free memory
+        front = max(1 << 20, j["est"])
+        cand = min(1 << 30, max(1 << 22, 4 * j["est"]))
         t = time.monotonic()
         while True:
-            r = gx.explore(m, property=j["prop"], por=int(j["por"]), q_ints=q)
+            r = gx.explore(m, property=j["prop"], por=int(j["por"]), q_ints=q, frontier_capacity=front,
+                           cand_capacity=cand)
+            if r["status"] == -10:  # a level larger than the frontier buffer
+                front *= 4
+                continue
             if not r["table_full"] or q >= (1 << 34):
                 break
             q = min(q * 4, 1 << 34)
@@ -152,7 +163,7 @@ def serve_workload(rate: float, seconds: float, seed: int = 0, server_kw: Option
     run = run_stream(jobs, times, server_kw, log)
     if log:
         log("serve: plain path for the verdict check")
-    plain = plain_path(jobs)
+    plain = plain_path(jobs, log)
     mismatches = check(jobs, run["results"], plain)
     fifo = fifo_latencies(times, [p["wall_s"] for p in plain])
     burst = run_stream(jobs, None, server_kw, log)
@@ -181,7 +192,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--rounds-per-step", type=int, default=16)
-    ap.add_argument("--table-gib", type=float, default=48.0)
+    ap.add_argument("--table-gib", type=float, default=96.0)
     args = ap.parse_args(argv)
     kw = {"slots": args.slots, "rounds_per_step": args.rounds_per_step,
           "table_bytes": int(args.table_gib * (1 << 30))}

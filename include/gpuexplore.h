@@ -194,18 +194,25 @@ typedef struct gx_batch_stats {
   uint64_t lookups;          /* FIND-OR-PUT operations, all instances */
   uint64_t states;           /* sum over instances */
   uint64_t transitions;      /* sum over instances */
+  uint64_t cand_entries;     /* successor-buffer entries scanned by FIND-OR-PUT (successors and the
+                                released tails of reservations) */
 } gx_batch_stats;
 
 void gx_batch_options_default(gx_batch_options *o);
 
 /* ------------------------------------------------------------------ */
 /* Serving: a continuous-batching request loop.  Jobs (model + property +
- * estimated state count) are queued by gx_server_submit; every
- * gx_server_step admits waiting jobs while an instance slot and a
- * contiguous run of visited-set pages for the job's estimated footprint are
- * free, replays `rounds_per_step` BFS rounds of all running jobs, and
- * retires finished ones.  A job whose partition fills up is restarted with
- * four times the pages, so results equal gx_explore's whatever the estimate.
+ * estimated state count) are queued by gx_server_submit into one of two
+ * lanes by their estimate (small jobs, large jobs); the lanes have their own
+ * slots, rounds and stream and run beside each other on the device.  Every
+ * gx_server_step works each lane whose last replay has finished (waiting up
+ * to about a millisecond for one): it retires finished jobs, admits waiting
+ * jobs while an instance slot and a contiguous run of visited-set pages for
+ * the job's estimated footprint are free, and starts a replay of
+ * `rounds_per_step` BFS rounds of the lane's running jobs without waiting
+ * for it.  A job whose partition (or its frontier, a quarter of the
+ * partition's slots) fills up is restarted with twice the pages, so results
+ * equal gx_explore's whatever the estimate.
  * Visited sets and frontiers of running jobs stay on the device between
  * steps.  No reference counterpart (the reference runs one model per
  * process). */
@@ -225,7 +232,9 @@ typedef struct gx_server_options {
   uint32_t model_words;      /* largest compiled model accepted (32-bit words) */
   uint32_t por_jobs;         /* 1: POR jobs accepted (each round also runs the POR kernel) */
   uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per round (0 = auto) */
-  uint32_t reserved;
+  uint32_t large_slots;      /* slots of the large-job lane (at most a quarter of `slots`) */
+  uint64_t large_states;     /* jobs estimated at this many states or more go to the large-job
+                                lane (0: every job to the small-job lane) */
 } gx_server_options;
 
 typedef struct gx_server_status {
