@@ -120,6 +120,8 @@ class ServerOptions(ctypes.Structure):
         ("insert_blocks", ctypes.c_uint32),
         ("large_slots", ctypes.c_uint32),
         ("large_states", ctypes.c_uint64),
+        ("large_cus", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

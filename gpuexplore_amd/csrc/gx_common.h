@@ -16,7 +16,13 @@ constexpr int kMaxProcs = 32;  // rules are N-bit masks packed in 32-bit ints (c
 constexpr int kMaxKeyWords = 5;  // 9 x 32 bits (cu:92) + 32-bit stamp fit in 5 x 63
 constexpr uint64_t kFlag = 1ull << 63;
 constexpr uint64_t kPayload = ~kFlag;
-constexpr int kHashFunctions = 8;  // buckets tried before the table is full (cu:88)
+// Buckets a key may probe before the table is full.  The reference tries 8
+// (NR_HASH_FUNCTIONS, cu:88) with buckets of 10 three-word slots; a 64-byte
+// bucket here holds 4 two-word slots, and with 8 probes a table reported
+// full at 37 % load (a 3.7e8-state model, 1e9 slots: a few keys in 1e6 found
+// 8 full buckets), so 16.  Extra probes are taken only by keys whose earlier
+// buckets are full.
+constexpr int kHashFunctions = 16;
 
 // Compiled model: offsets (in 32-bit words) of the arrays inside one
 // contiguous buffer that a workgroup stages into LDS.

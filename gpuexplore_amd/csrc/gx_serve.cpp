@@ -1,8 +1,8 @@
 // gx_serve.cpp -- continuous-batching model-checking server.
 //
 // A server owns fixed device pools: a visited set of `table_bytes` cut into
-// pages, frontier pages paired with the table pages (a quarter of a page's
-// slots each), and a model region per instance slot.  Jobs are served in two
+// pages, frontier pages paired with the table pages (half a page's slots
+// each), and a model region per instance slot.  Jobs are served in two
 // lanes by estimated footprint: small jobs and large ones (an estimate of at
 // least `large_states` states).  Each lane has its own instance slots,
 // control block, successor buffer, captured rounds and stream, so the two
@@ -151,7 +151,15 @@ int create_lane(gx_server *s, int k, uint32_t slot0, uint32_t nslots, uint64_t c
   SV_TRY(L.cblk.alloc((cand_cap / 64 + 1) * 4));
   SV_TRY(L.def0.alloc(max_chunks * 8));
   SV_TRY(L.def1.alloc(max_chunks * 8));
-  SV_TRY(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+  if (k == 1 && s->o.large_cus < cus) {
+    // the large-job lane's kernels run on the first large_cus compute units
+    // only, so the small-job lane's rounds always find free ones
+    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+    for (uint32_t c = 0; c < s->o.large_cus; c++) mask[c / 32] |= 1u << (c % 32);
+    SV_TRY(hipExtStreamCreateWithCUMask(&L.st, static_cast<uint32_t>(mask.size()), mask.data()));
+  } else {
+    SV_TRY(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+  }
   SV_TRY(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
   SV_TRY(hipHostMalloc(reinterpret_cast<void **>(&L.h_inst), sizeof(BatchInst) * nslots, hipHostMallocDefault));
   SV_TRY(hipHostMalloc(reinterpret_cast<void **>(&L.h_ctrl), sizeof(BatchCtrl), hipHostMallocDefault));
@@ -237,10 +245,11 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   s->page_buckets = std::max<uint64_t>(1, o.page_bytes / bb);
   s->npages = std::max<uint64_t>(1, o.table_bytes / (s->page_buckets * bb));
   s->page_entries = s->page_buckets * s->spb;
-  // frontier entries per page: a quarter of the page's slots (a partition
-  // sized for twice its states holds a level of half its states; a job whose
-  // level overflows restarts with more pages, like a full partition)
-  s->page_front = std::max<uint64_t>(64, s->page_entries / 4);
+  // frontier entries per page: half the page's slots, so a partition sized
+  // for twice its estimate holds a level of every estimated state (a 24-process
+  // POR job keeps up to two thirds of its states in its last level); a job
+  // whose level overflows restarts with more pages, like a full partition
+  s->page_front = std::max<uint64_t>(64, s->page_entries / 2);
   s->model_words = o.model_words;
   int dev = 0;
   SV_TRY(hipGetDevice(&dev));
@@ -249,6 +258,11 @@ extern "C" int gx_server_create(const gx_server_options *opt, gx_server **out) {
   const uint32_t cus = static_cast<uint32_t>(prop.multiProcessorCount);
   if (!s->o.blocks) s->o.blocks = cus * (2048 / s->o.threads);
   if (!s->o.insert_blocks) s->o.insert_blocks = cus * 16;
+  // the large-job lane leaves an eighth of the CUs to the small-job lane by
+  // default: a 20 jobs/s stream of 24-process POR jobs beside 20-process
+  // 3e8-state jobs had a POR-job p50 of 236 ms sharing every CU, 29 ms with
+  // 224 of 256 CUs for the large jobs (192: 24 ms, but the large jobs slow)
+  if (!s->o.large_cus) s->o.large_cus = cus - cus / 8;
   const uint64_t S = o.slots, W = s->W;
   const uint64_t front_total = s->npages * s->page_front;
   SV_TRY(s->inst.alloc(sizeof(BatchInst) * S));

@@ -55,9 +55,9 @@ def pct(xs: List[float], q: float) -> float:
     return s[min(len(s) - 1, max(0, int(math.ceil(q * len(s))) - 1))]
 
 
-# Server pools for the stream: a 96 GiB visited set (frontier pools of a
-# quarter of its slots each), enough for six of the 5e8-state jobs at their
-# estimate beside the POR jobs.
+# Server pools for the stream: a 96 GiB visited set (frontier pools of half
+# its slots each), enough for six of the 5e8-state jobs at their estimate
+# beside the POR jobs.
 SERVER_KW = {"table_bytes": 96 << 30, "page_bytes": 4 << 20}
 
 
@@ -154,13 +154,27 @@ def check(jobs: List[dict], served: List[dict], plain: List[dict]) -> int:
 
 
 def serve_workload(rate: float, seconds: float, seed: int = 0, server_kw: Optional[dict] = None,
-                   log=None) -> Dict:
+                   log=None, check: bool = True) -> Dict:
+    """The stream through one server, then (with ``check``) every job alone
+    through the plain path, as the verdict reference and the one-at-a-time
+    baseline, and the whole stream submitted at once (saturation)."""
     count = max(1, int(round(rate * seconds)))
     jobs = make_jobs(count, seed * 1000003)
     times = arrivals(count, rate, seed)
     if log:
         log(f"serve: {count} jobs over {times[-1]:.1f}s (rate {rate}/s)")
     run = run_stream(jobs, times, server_kw, log)
+    if not check:
+        return {"jobs": count, "rate_per_s": rate, "seconds": seconds, "states": run["states"],
+                "p50_time_to_verdict_s": run["p50_s"], "p99_time_to_verdict_s": run["p99_s"],
+                "stream_states_per_s": run["states"] / run["span_s"], "restarts": run["restarts"],
+                "by_kind": {k: {"p50_s": pct([run["results"][i]["latency_s"] for i, j in enumerate(jobs)
+                                              if j["por"] == (k == "por_jobs")], 0.5),
+                                "p99_s": pct([run["results"][i]["latency_s"] for i, j in enumerate(jobs)
+                                              if j["por"] == (k == "por_jobs")], 0.99),
+                                "restarts": sum(run["results"][i]["restarts"] for i, j in enumerate(jobs)
+                                                if j["por"] == (k == "por_jobs"))}
+                            for k in ("por_jobs", "large_plain_jobs")}}
     if log:
         log("serve: plain path for the verdict check")
     plain = plain_path(jobs, log)
@@ -169,7 +183,15 @@ def serve_workload(rate: float, seconds: float, seed: int = 0, server_kw: Option
     burst = run_stream(jobs, None, server_kw, log)
     mismatches += check(jobs, burst["results"], plain)
     plain_total = sum(p["wall_s"] for p in plain)
+    by_kind = {}
+    for kind, sel in (("por_jobs", lambda j: j["por"]), ("large_plain_jobs", lambda j: not j["por"])):
+        idx = [i for i, j in enumerate(jobs) if sel(j)]
+        lat = [run["results"][i]["latency_s"] for i in idx]
+        by_kind[kind] = {"jobs": len(idx), "p50_s": pct(lat, 0.50), "p99_s": pct(lat, 0.99),
+                         "states": sum(run["results"][i]["states"] for i in idx),
+                         "restarts": sum(run["results"][i]["restarts"] for i in idx)}
     return {
+        "by_kind": by_kind,
         "jobs": count, "rate_per_s": rate, "seconds": seconds,
         "states": run["states"],
         "p50_time_to_verdict_s": run["p50_s"], "p99_time_to_verdict_s": run["p99_s"],
@@ -193,13 +215,15 @@ def main(argv=None) -> int:
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--rounds-per-step", type=int, default=16)
     ap.add_argument("--table-gib", type=float, default=96.0)
+    ap.add_argument("--large-cus", type=int, default=0, help="compute units of the large-job lane (0: all)")
+    ap.add_argument("--no-check", action="store_true", help="the stream only: no plain-path check, no burst")
     args = ap.parse_args(argv)
-    kw = {"slots": args.slots, "rounds_per_step": args.rounds_per_step,
-          "table_bytes": int(args.table_gib * (1 << 30))}
+    kw = dict(SERVER_KW, slots=args.slots, rounds_per_step=args.rounds_per_step,
+              table_bytes=int(args.table_gib * (1 << 30)), large_cus=args.large_cus)
     out = serve_workload(args.rate, args.seconds, args.seed, kw,
-                         log=lambda s: print(s, file=sys.stderr, flush=True))
+                         log=lambda s: print(s, file=sys.stderr, flush=True), check=not args.no_check)
     print(json.dumps(out))
-    return 1 if out["verdict_mismatches"] else 0
+    return 1 if out.get("verdict_mismatches") else 0
 
 
 if __name__ == "__main__":

@@ -210,9 +210,9 @@ void gx_batch_options_default(gx_batch_options *o);
  * jobs while an instance slot and a contiguous run of visited-set pages for
  * the job's estimated footprint are free, and starts a replay of
  * `rounds_per_step` BFS rounds of the lane's running jobs without waiting
- * for it.  A job whose partition (or its frontier, a quarter of the
- * partition's slots) fills up is restarted with twice the pages, so results
- * equal gx_explore's whatever the estimate.
+ * for it.  A job whose partition (or its frontier, half the partition's
+ * slots) fills up is restarted with twice the pages, so results equal
+ * gx_explore's whatever the estimate.
  * Visited sets and frontiers of running jobs stay on the device between
  * steps.  No reference counterpart (the reference runs one model per
  * process). */
@@ -235,6 +235,10 @@ typedef struct gx_server_options {
   uint32_t large_slots;      /* slots of the large-job lane (at most a quarter of `slots`) */
   uint64_t large_states;     /* jobs estimated at this many states or more go to the large-job
                                 lane (0: every job to the small-job lane) */
+  uint32_t large_cus;        /* compute units the large-job lane's kernels may use (0 = seven
+                                eighths; the device's count or more = all); the others stay free
+                                for the small-job lane */
+  uint32_t reserved;
 } gx_server_options;
 
 typedef struct gx_server_status {
