@@ -203,9 +203,10 @@ class OracleModel:
 
 
 def explore(path_noext: str, prop: int = PROP_NONE, por: bool = False, proviso: bool = False,
-            max_states: int = 0, seed: int = 0, method: int = 0) -> dict:
-    return OracleModel(path_noext, por).explore(prop, por, proviso, max_states, seed, method)
+            max_states: int = 0, seed: int = 0, method: int = 0, expected_states: int = 0) -> dict:
+    return OracleModel(path_noext, por).explore(prop, por, proviso, max_states, seed, method, expected_states)
 
 
-def explore_mt(path_noext: str, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0) -> dict:
-    return OracleModel(path_noext, False).explore_mt(threads, prop, max_states, seed)
+def explore_mt(path_noext: str, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0,
+               expected_states: int = 0) -> dict:
+    return OracleModel(path_noext, False).explore_mt(threads, prop, max_states, seed, expected_states)

@@ -1,6 +1,6 @@
-"""Full-size workloads on the GPU: config2 against the oracle's committed
-counts, config3 under POR (with and without the cycle proviso) against the
-oracle run live."""
+"""Full-size workloads on the GPU: config2 and config3 unreduced against the
+oracle's committed full counts, config3 under POR (with and without the
+cycle proviso) against the oracle run live."""
 
 import json
 import os
@@ -24,8 +24,8 @@ def test_config2_full_size(gpu, tmp_path):
     g = json.load(open(os.path.join(GOLDEN, "config2_counts.json")))["runs"]["plain"]
     p = str(tmp_path / "config2")
     gen.config2(1).write(p)
-    say("config2 full size")
-    r = gpu.explore(gpu.Model.load(p), q_ints=1 << 32)
+    say("config2 full size, 2^34-int visited set")
+    r = gpu.explore(gpu.Model.load(p), q_ints=1 << 34)
     assert r["table_full"] == 0
     assert (r["states"], r["transitions"], r["levels"], bool(r["violation"])) == \
         (g["states"], g["transitions"], g["levels"], g["violation"])
@@ -44,23 +44,19 @@ def test_config3_por(gpu, oracle_mod, tmp_path, prop, proviso):
         (o["states"], o["transitions"], o["levels"], o["violation"])
 
 
-def test_config3_no_por(gpu, tmp_path):
-    """the unreduced space (945M states) against the oracle's counts of its
-    first levels: the oracle cannot hold the whole set in host memory here"""
-    path = os.path.join(GOLDEN, "config3_counts.json")
-    if not os.path.exists(path):
-        pytest.skip("no full-size counts for config3")
-    runs = json.load(open(path))["runs"]
+@pytest.mark.parametrize("name", ["plain", "-d"])
+def test_config3_no_por(gpu, tmp_path, name):
+    """the unreduced space (944.9 M states, 15.25 G transitions) with and
+    without the deadlock check, against the oracle's committed full counts
+    (tests/golden/make_full_counts.py, the multi-threaded oracle with a
+    presized visited set)"""
+    g = json.load(open(os.path.join(GOLDEN, "config3_counts.json")))["runs"][name]
+    assert not g.get("truncated")
     p = str(tmp_path / "config3")
     gen.config3(3).write(p)
-    for name, g in runs.items():
-        if "--por" in name:
-            continue
-        prop = 1 if "-d" in name.split() else 0
-        say("config3 no POR", name)
-        r = gpu.explore(gpu.Model.load(p), property=prop, q_ints=1 << 34,
-                        max_states=g.get("max_states", 0))
-        assert r["table_full"] == 0
-        assert bool(r["truncated"]) == g.get("truncated", False) or not g.get("max_states")
-        assert (r["states"], r["transitions"], r["levels"], bool(r["violation"])) == \
-            (g["states"], g["transitions"], g["levels"], g["violation"])
+    prop = 1 if name == "-d" else 0
+    say("config3 no POR", name)
+    r = gpu.explore(gpu.Model.load(p), property=prop, q_ints=1 << 34)
+    assert r["table_full"] == 0 and r["truncated"] == 0
+    assert (r["states"], r["transitions"], r["levels"], bool(r["violation"])) == \
+        (g["states"], g["transitions"], g["levels"], g["violation"])

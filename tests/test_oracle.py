@@ -3,6 +3,7 @@ force, and an independent pure-Python enumerator on small models."""
 
 import json
 import os
+import subprocess
 import random
 
 import pytest
@@ -190,6 +191,55 @@ def test_multithreaded_oracle_equals_sequential(oracle_mod, model_dir, threads):
     b = oracle_mod.explore_mt(p, threads, 0, max_states=5000)
     assert (a["states"], a["transitions"], a["levels"], a["truncated"]) == \
         (b["states"], b["transitions"], b["levels"], b["truncated"])
+
+
+@pytest.mark.parametrize("expect", [1000, 100000])
+def test_presized_level_free_set(oracle_mod, model_dir, expect):
+    """the visited set presized (no growth) and without a level array (plain
+    and POR without the proviso), in the sequential and the multi-threaded
+    BFS, gives the growing set's counts; the proviso keeps its levels"""
+    for name, net in small_models()[:6] + [("c1", gen.config1(3))]:
+        p = os.path.join(model_dir, name)
+        net.write(p)
+        for prop in (0, 1, 2):
+            for por, prov in ((False, False), (True, False), (True, True)):
+                a = oracle_mod.explore(p, prop, por, prov)
+                b = oracle_mod.explore(p, prop, por, prov, expected_states=expect)
+                assert (a["states"], a["transitions"], a["levels"], a["violation"]) == \
+                    (b["states"], b["transitions"], b["levels"], b["violation"]), (name, prop, por, prov)
+            a = oracle_mod.explore(p, prop)
+            c = oracle_mod.explore_mt(p, 4, prop, expected_states=expect)
+            assert (a["states"], a["transitions"], a["levels"], a["violation"]) == \
+                (c["states"], c["transitions"], c["levels"], c["violation"]), (name, prop)
+
+
+def test_multithreaded_oracle_under_tsan(tmp_path):
+    """the multi-threaded BFS (sharded visited set behind mutexes, frontier
+    chunks taken from an atomic counter, per-thread next levels joined
+    between levels) has no data race ThreadSanitizer can see"""
+    oracle_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["make", "-s", "-C", oracle_dir, "tsan"], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("no ThreadSanitizer runtime for gcc here: " + r.stderr[-200:])
+    cli = os.path.join(oracle_dir, "_build", "gx_oracle_cli_tsan")
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
+    for name, net, flags in (("c1", gen.config1(1), []), ("c1d", gen.config1(2), ["-d"]),
+                             ("mon", small_models()[2][1], ["-p"])):
+        p = str(tmp_path / name)
+        net.write(p)
+        out = subprocess.run([cli, p, "--threads", "8", "--expect", "5000"] + flags, capture_output=True, text=True,
+                             env=env, timeout=600)
+        assert out.returncode == 0 and "ThreadSanitizer" not in out.stderr, (name, out.stderr[-2000:])
+        seq = oracle_mod_explore(p, flags)
+        got = dict(tok.split("=") for tok in out.stdout.split())
+        assert (int(got["states"]), int(got["transitions"]), int(got["levels"])) == seq, name
+
+
+def oracle_mod_explore(p, flags):
+    from oracle import oracle
+    prop = 1 if "-d" in flags else (2 if "-p" in flags else 0)
+    o = oracle.explore(p, prop)
+    return (o["states"], o["transitions"], o["levels"])
 
 
 @pytest.mark.parametrize("name", [n for n, _ in __import__("edge_models").edge_models()])
