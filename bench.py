@@ -128,6 +128,7 @@ def main() -> int:
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--threads", type=int, default=0, help="successor-kernel workgroup size, 0 = library default")
     ap.add_argument("--insert-blocks", type=int, default=0, help="FIND-OR-PUT workgroups per launch, 0 = sized per level")
+    ap.add_argument("--dedup", type=int, default=0, help="FIND-OR-PUT duplicate filter (1 on, 0 off)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
     ap.add_argument("--no-por", action="store_true", help="skip the config 3 partial-order reduction runs")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -161,7 +162,7 @@ def main() -> int:
 
     ex = gx.Explorer(model, q_ints=1 << args.q_log2, property=gx.PROP_NONE,
                      expand_kernel=args.expand_kernel, bucket_bytes=args.bucket_bytes, threads=args.threads,
-                     insert_blocks=args.insert_blocks)
+                     insert_blocks=args.insert_blocks, dedup=args.dedup)
     for i in range(args.warmup):
         r = ex.run()
         print(f"warmup {i}: states={r['states']} t_explore={r['t_explore_s']:.3f}s", file=sys.stderr, flush=True)
@@ -279,6 +280,7 @@ def main() -> int:
             "bucket_bytes": args.bucket_bytes,
             "threads": args.threads,
             "insert_blocks": args.insert_blocks,
+            "dedup": args.dedup,
             "expand_kernel": ["auto", "process-groups", "state-per-lane", "fused"][args.expand_kernel],
         },
         "roofline": {
@@ -303,6 +305,9 @@ def main() -> int:
             "lookups_vs_random64": lookups / t_hash / rand64.value if rand64.value else None,
         },
         "hash_phase": {
+            "candidates": sum(r["transitions"] for r in runs) // args.steps,
+            "lookups": lookups // args.steps,
+            "dedup_share": 1.0 - lookups / max(1, sum(r["transitions"] for r in runs)),
             "lookups_per_s": lookups / t_hash,
             "inserts_per_s": inserts / t_hash,
             # each insert claims its slot with one 64-bit atomicCAS; attempts

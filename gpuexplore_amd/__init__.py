@@ -50,6 +50,8 @@ class Options(ctypes.Structure):
         ("expand_kernel", ctypes.c_uint32),
         ("insert_blocks", ctypes.c_uint32),
         ("por_method", ctypes.c_uint32),
+        ("dedup", ctypes.c_uint32),
+        ("reserved2", ctypes.c_uint32),
     ]
 
 

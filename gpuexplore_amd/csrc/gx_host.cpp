@@ -141,6 +141,7 @@ extern "C" void gx_options_default(gx_options *o) {
   o->kernel_iters = 1;
   o->bucket_bytes = 0;
   o->hash_seed = 0x5eed5eed5eedull;
+  o->dedup = 0;  // on a 20-process ring it drops 15 % of the lookups, but not the kernel's time
 }
 
 extern "C" const char *gx_strerror(int s) {
@@ -413,6 +414,7 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   ia.state_bits = H.state_bits;
   ia.is_new = nullptr;
   ia.append = 1;
+  ia.dedup = opt->dedup ? 1 : 0;
 
   FusedArgs fa{};
   fa.model = e->d_model.as<uint32_t>();

@@ -65,7 +65,13 @@ struct InsertArgs {
   int append;       // append new keys to front[cur ^ 1]
   int set_flag;     // keys come from a caller: set bit 63 of every word (a caller's
                     // zero word would otherwise read as a released or unpublished one)
+  int dedup;        // skip candidates equal to an earlier one of the same or the previous
+                    // 64-candidate step of the wave (needs append and no is_new)
 };
+
+// LDS tag table of the FIND-OR-PUT duplicate filter: per wave, kDedupSlots
+// words of (step << 6 | lane), indexed by the key hash's low bits
+constexpr uint32_t kDedupSlots = 128;
 
 // Resets a control block for a FIND-OR-PUT pass of n candidates with a
 // kernel (no host copy, no memset): what a caller that cannot allocate or

@@ -403,6 +403,8 @@ void k_insert(InsertArgs a) {
   const int cg = lane / LPB;
   const int gbase = cg * LPB;
   uint64_t *stage = stage_all + static_cast<size_t>(threadIdx.x >> 6) * (kStage * W);
+  uint32_t *dtag = reinterpret_cast<uint32_t *>(stage_all + static_cast<size_t>(blockDim.x >> 6) * (kStage * W)) +
+                   static_cast<size_t>(threadIdx.x >> 6) * kDedupSlots;
   uint32_t staged = 0;  // wave-uniform
   const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
@@ -482,6 +484,22 @@ void k_insert(InsertArgs a) {
     // hoisted out of the probe loop and computed for every candidate).
     static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
     const uint64_t RUN = CPW * kInsertRun;
+    // Duplicate filter (a.dedup): the successors of commuting moves repeat
+    // within a few dozen candidates, so a candidate equal to another of the
+    // same step, or of the wave's previous step, is not looked up -- the
+    // other lane's lookup (this step) or the finished one (last step) stands
+    // for it.  Lanes tag an LDS slot chosen by the key hash with
+    // (step << 6 | lane) and compare keys through cross-lane reads, so only
+    // equal keys are dropped; slot collisions of different keys just look
+    // both up.
+    uint32_t sid = 1;  // step of this wave, wave-uniform
+    uint64_t key_prev[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) key_prev[j] = 0;
+    if (a.dedup) {
+      for (uint32_t i = lane; i < kDedupSlots; i += 64) dtag[i] = 0;
+      wave_sync();
+    }
     for (uint64_t r0 = gw * RUN, c0 = r0; c0 < n;) {
       const uint64_t cl = c0 + lane;
       const uint64_t cstep = c0;
@@ -494,9 +512,33 @@ void k_insert(InsertArgs a) {
       uint64_t key_l[W];
 #pragma unroll
       for (int j = 0; j < W; j++) key_l[j] = cl < n ? (__builtin_nontemporal_load(a.cand + cl * W + j) | fl) : 0;
-      const uint64_t act_m = ballot(key_l[0] != 0);
+      uint64_t act_m = ballot(key_l[0] != 0);
       const uint64_t h_l = key_hash<W>(key_l, a.table.seed);
       const uint64_t b_l = bucket_of(h_l, 0, a.table.nbuckets);
+      if (a.dedup) {
+        const bool act = (act_m >> lane) & 1ull;
+        const uint32_t slot = static_cast<uint32_t>(h_l) & (kDedupSlots - 1u);
+        auto equal_to = [&](const uint64_t (&other)[W], int src) {
+          bool eq = true;
+#pragma unroll
+          for (int j = 0; j < W; j++) eq = eq && (__shfl(other[j], src, 64) == key_l[j]);
+          return eq;
+        };
+        // a lane of the previous step with this slot and an equal key
+        const uint32_t old = dtag[slot];
+        bool dup = equal_to(key_prev, static_cast<int>(old & 63u)) && act && (old >> 6) == sid - 1u;
+        wave_sync();
+        if (act && !dup) dtag[slot] = (sid << 6) | static_cast<uint32_t>(lane);
+        wave_sync();
+        // the lane of this step that holds the slot, when its key is equal
+        const uint32_t cur = dtag[slot];
+        const bool eq_cur = equal_to(key_l, static_cast<int>(cur & 63u));
+        dup = dup || (act && (cur >> 6) == sid && static_cast<int>(cur & 63u) != lane && eq_cur);
+        act_m = ballot(act && !dup);
+#pragma unroll
+        for (int j = 0; j < W; j++) key_prev[j] = key_l[j];
+        sid++;
+      }
 #pragma unroll 1
       for (int s = 0; s < LPB; s++) {
         const int src = s * CPW + cg;
@@ -686,7 +728,7 @@ hipError_t launch_expand(const ExpandArgs &a, int mode, int key_words, int slot_
 
 template <int W, int WT, bool STAMP>
 static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
-  const size_t shm = static_cast<size_t>(threads / 64) * kStage * W * 8;
+  const size_t shm = static_cast<size_t>(threads / 64) * (kStage * W * 8 + kDedupSlots * 4);
   switch (bw) {
     case 8:
       if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break; }

@@ -285,3 +285,33 @@ def test_stop_with_levels_enqueued_ahead(gpu, oracle_mod, model_dir, kiters):
     say("ahead truncated", kiters)
     b = gpu_run(gpu, p, 0, kernel_iters=kiters, max_states=2000)
     assert same(a, b) and a["truncated"] and b["truncated"], (kiters, a, b)
+
+
+def test_dedup_filter_invariance(gpu, oracle_mod, model_dir):
+    """The FIND-OR-PUT duplicate filter changes no count (the reference's
+    LDS cache is semantics-neutral, cu:210-265): on and off give the same
+    states, transitions, levels and verdicts in every mode, and with it on
+    fewer candidates reach the visited set."""
+    from test_oracle import small_models
+    nets = small_models()[:5] + [("c1", gen.config1(1)), ("c3por", gen.config3(3))]
+    for name, net in nets:
+        p = os.path.join(model_dir, name)
+        net.write(p)
+        modes = [] if name == "c3por" else [(0, False, False), (1, False, False), (2, False, False)]
+        modes += [(1, True, False), (0, True, True)]  # config 3 unreduced does not fit this table
+        for prop, por, prov in modes:
+            say("dedup", name, prop, por, prov)
+            runs = [gpu.explore(gpu.Model.load(p, por=por), property=prop, por=int(por), cycle_proviso=int(prov),
+                                q_ints=1 << 26, dedup=d) for d in (0, 1)]
+            keys = ("states", "transitions", "levels", "violation")
+            assert runs[0]["table_full"] == runs[1]["table_full"] == 0, name
+            assert [runs[0][k] for k in keys] == [runs[1][k] for k in keys], (name, prop, por, prov)
+            assert runs[1]["lookups"] <= runs[0]["lookups"]
+    # a large plain level: the filter takes out a share of the lookups
+    p = os.path.join(model_dir, "c2")
+    gen.config2(1).write(p)
+    runs = [gpu.explore(gpu.Model.load(p), q_ints=1 << 30, max_states=20_000_000, dedup=d) for d in (0, 1)]
+    assert [runs[0][k] for k in ("states", "transitions", "levels")] == \
+        [runs[1][k] for k in ("states", "transitions", "levels")]
+    assert runs[0]["lookups"] == runs[0]["transitions"] + 1  # + the initial state's
+    assert runs[1]["lookups"] < 0.97 * runs[0]["lookups"], (runs[0]["lookups"], runs[1]["lookups"])
