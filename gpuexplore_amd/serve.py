@@ -142,7 +142,7 @@ def fifo_latencies(times: List[float], service: List[float]) -> List[float]:
     return lat
 
 
-def check(jobs: List[dict], served: List[dict], plain: List[dict]) -> int:
+def count_mismatches(jobs: List[dict], served: List[dict], plain: List[dict]) -> int:
     bad = 0
     for j, s, p in zip(jobs, served, plain):
         a = (s["states"], s["transitions"], s["levels"], s["violation"], s["table_full"], s["status"])
@@ -178,10 +178,10 @@ def serve_workload(rate: float, seconds: float, seed: int = 0, server_kw: Option
     if log:
         log("serve: plain path for the verdict check")
     plain = plain_path(jobs, log)
-    mismatches = check(jobs, run["results"], plain)
+    mismatches = count_mismatches(jobs, run["results"], plain)
     fifo = fifo_latencies(times, [p["wall_s"] for p in plain])
     burst = run_stream(jobs, None, server_kw, log)
-    mismatches += check(jobs, burst["results"], plain)
+    mismatches += count_mismatches(jobs, burst["results"], plain)
     plain_total = sum(p["wall_s"] for p in plain)
     by_kind = {}
     for kind, sel in (("por_jobs", lambda j: j["por"]), ("large_plain_jobs", lambda j: not j["por"])):
