@@ -124,7 +124,7 @@ def main() -> int:
     ap.add_argument("--cpu-single-states", type=int, default=10_000_000,
                     help="one-thread CPU baseline: stop after the level reaching this many states")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane, 3 fused level")
+    ap.add_argument("--expand-kernel", type=int, default=0, help="0 auto, 1 process groups, 2 one state per lane")
     ap.add_argument("--bucket-bytes", type=int, default=0, help="0 auto (64 for 2-word slots)")
     ap.add_argument("--threads", type=int, default=0, help="successor-kernel workgroup size, 0 = library default")
     ap.add_argument("--insert-blocks", type=int, default=0, help="FIND-OR-PUT workgroups per launch, 0 = sized per level")
@@ -281,7 +281,7 @@ def main() -> int:
             "threads": args.threads,
             "insert_blocks": args.insert_blocks,
             "dedup": args.dedup,
-            "expand_kernel": ["auto", "process-groups", "state-per-lane", "fused"][args.expand_kernel],
+            "expand_kernel": ["auto", "process-groups", "state-per-lane"][args.expand_kernel],
         },
         "roofline": {
             "bound": "hbm",

@@ -79,9 +79,6 @@ uint32_t expand_grid(uint32_t cus, uint32_t threads, uint32_t max_blocks, double
   return static_cast<uint32_t>(std::min<double>(max_blocks, std::max<double>(cus, blocks)));
 }
 
-// candidate entries per wave in the fused level kernel
-constexpr uint64_t kFusedSlice = 2048;
-
 uint32_t round_threads(uint32_t t) {
   if (t == 0) t = 256;
   t = (t + 63) / 64 * 64;
@@ -170,8 +167,7 @@ struct gx_explorer {
   const gx_model *m = nullptr;
   gx_options opt{};
   int W = 1, WT = 1, BW = 16, mode = kModePlain;
-  bool stamp = false, lds = false, per_state = false, fused = false;
-  uint64_t cw = 0;  // fused: candidate entries per wave slice
+  bool stamp = false, lds = false, per_state = false;
   uint32_t threads = 256, blocks = 1, cus = 1;
   uint32_t ins_blocks = 1;  // FIND-OR-PUT launch: ins_blocks x kInsertThreads
   uint64_t nbuckets = 0, table_bytes = 0, front_cap = 0, cand_cap = 0;
@@ -324,33 +320,13 @@ extern "C" int gx_explorer_create(const gx_model *m, const gx_options *opt, gx_e
   const size_t model_b = ((H.total_words + 3u) & ~3u) * 4u;
   const size_t sec_b = ((H.total_words - H.tps_begin + 3u) & ~3u) * 4u;  // state-per-lane section
   const size_t tps_b = expand_tps_scratch_bytes(e->threads, H.nprocs, H.nrows < 0xFFFFu);
+  if (opt->expand_kernel > 2) return GX_E_ARG;
   e->per_state = e->mode == kModePlain && H.lab64 && opt->expand_kernel != 1 && tps_b <= lds_limit;
   if (opt->expand_kernel == 2 && !e->per_state) return GX_E_ARG;
   if (e->per_state)
     e->lds = sec_b + tps_b <= lds_limit;
   else
     e->lds = model_b + expand_scratch_bytes(e->threads) <= 48 * 1024;
-  // fused level (successors + FIND-OR-PUT in one kernel, 256 threads): each
-  // wave owns a slice of the candidate buffer
-  e->fused = opt->expand_kernel == 3;
-  if (e->fused) {
-    if (!e->per_state) return GX_E_ARG;
-    e->threads = 256;
-    e->blocks = opt->blocks ? std::max<uint32_t>(1, opt->blocks) : e->cus * 32;
-    const uint64_t waves = static_cast<uint64_t>(e->blocks) * 4;
-    if (opt->cand_capacity) {  // explicit: slices of what it allows
-      e->cw = std::min<uint64_t>(kFusedSlice, e->cand_cap / waves);
-      if (e->cw < 64) return GX_E_CAPACITY;
-    } else {
-      e->cw = kFusedSlice;
-      if (e->cand_cap < waves * e->cw) {
-        e->cand_cap = waves * e->cw;  // alloc() releases the smaller buffer
-        GX_TRY(e->d_cand.alloc(e->cand_cap * W * 8));
-      }
-    }
-    e->lds = sec_b + fused_scratch_bytes(256, H.nprocs, W) <= lds_limit;
-    if (!e->lds && fused_scratch_bytes(256, H.nprocs, W) > lds_limit) return GX_E_LIMIT;
-  }
   *out = guard.release();
   return GX_OK;
 }
@@ -416,18 +392,6 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   ia.append = 1;
   ia.dedup = opt->dedup ? 1 : 0;
 
-  FusedArgs fa{};
-  fa.model = e->d_model.as<uint32_t>();
-  fa.hdr = H;
-  fa.ctrl = e->d_ctrl.as<Ctrl>();
-  fa.front[0] = e->d_front0.as<uint64_t>();
-  fa.front[1] = e->d_front1.as<uint64_t>();
-  fa.front_cap = e->front_cap;
-  fa.cand = e->d_cand.as<uint64_t>();
-  fa.cw = e->cw;
-  fa.table = T;
-  fa.property = opt->property;
-
   // initial state: FIND-OR-PUT of s0 into frontier 0 with stamp 0
   GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, 1, 64, st));
   GX_TRY(launch_advance(ia.ctrl, st));
@@ -462,14 +426,6 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
     for (uint32_t it = 0; it < kiters; it++) {
       hipEvent_t a0 = e->evs[k].next(), a1 = e->evs[k].next(), b0 = e->evs[k].next(), b1 = e->evs[k].next();
       if (!a0 || !a1 || !b0 || !b1) return GX_E_HIP;
-      if (e->fused) {  // one launch per level: its time is hash time
-        GX_TRY(hipEventRecord(b0, st));
-        GX_TRY(launch_fused(fa, W, BW, e->lds, exp_blocks, 256, st));
-        GX_TRY(hipEventRecord(b1, st));
-        GX_TRY(launch_advance(ia.ctrl, st));
-        b.ins.push_back({b0, b1});
-        continue;
-      }
       GX_TRY(hipEventRecord(a0, st));
       GX_TRY(launch_expand(ea, e->mode, W, WT, e->lds, exp_blocks, e->threads, st));
       GX_TRY(hipEventRecord(a1, st));

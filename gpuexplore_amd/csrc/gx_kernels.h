@@ -31,28 +31,6 @@ inline size_t expand_tps_scratch_bytes(uint32_t threads, uint32_t nprocs, bool r
   return static_cast<size_t>(threads / 64) * nprocs * 64 * (rows16 ? 2 : 4) + 16;
 }
 
-// Fused level (plain method, one state per lane): every wave expands its
-// batch into a private slice of `cw` candidate entries and FIND-OR-PUTs them
-// right away, so successor generation (ALU) of some waves overlaps the table
-// accesses (HBM) of others and candidates never leave the caches.
-struct FusedArgs {
-  const uint32_t *model;
-  ModelHdr hdr;
-  Ctrl *ctrl;
-  uint64_t *front[2];
-  uint64_t front_cap;
-  uint64_t *cand;  // per-wave slices of cw entries (W words each)
-  uint64_t cw;
-  Table table;
-  int property;
-};
-
-// LDS bytes k_fused needs besides the staged model.
-size_t fused_scratch_bytes(uint32_t threads, uint32_t nprocs, int key_words);
-
-hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, bool lds, uint32_t blocks,
-                        uint32_t threads, hipStream_t s);
-
 struct InsertArgs {
   Ctrl *ctrl;
   const uint64_t *cand;
@@ -68,6 +46,10 @@ struct InsertArgs {
   int dedup;        // skip candidates equal to an earlier one of the same or the previous
                     // 64-candidate step of the wave (needs append and no is_new)
 };
+
+// compile-time options of the FIND-OR-PUT kernel (InsertArgs.dedup, .set_flag)
+constexpr int kInsDedup = 1;
+constexpr int kInsSetFlag = 2;
 
 // LDS tag table of the FIND-OR-PUT duplicate filter: per wave, kDedupSlots
 // words of (step << 6 | lane), indexed by the key hash's low bits

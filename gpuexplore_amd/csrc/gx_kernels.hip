@@ -200,181 +200,6 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused level: successors of a 64-state batch go to the wave's own slice of
-// the candidate buffer and are offered to the table by the same wave (groups
-// of BW/2 lanes per candidate, as in k_insert).  A batch whose successors
-// exceed the slice is split into windows of whole lanes; a single state with
-// more than `cw` successors stops the exploration with an overflow.
-constexpr int kFusedStage = 128;
-
-size_t fused_scratch_bytes(uint32_t threads, uint32_t nprocs, int key_words) {
-  return static_cast<size_t>(threads / 64) * (nprocs * 64 * 4 + kFusedStage * key_words * 8) + 16;
-}
-
-template <int W, int BW, bool LDS>
-__global__ __launch_bounds__(256) void k_fused(FusedArgs a) {
-  extern __shared__ uint32_t smem[];
-  const ModelHdr H = LDS ? tps_rebased(a.hdr) : a.hdr;
-  Ctrl *C = a.ctrl;
-  const uint32_t sec = a.hdr.total_words - a.hdr.tps_begin;
-  if (LDS) {
-    stage_words(smem, a.model + a.hdr.tps_begin, sec);
-  }
-  const uint32_t *M = LDS ? smem : a.model;
-  const uint32_t xoff = LDS ? ((sec + 3u) & ~3u) : 0u;
-  const int N = static_cast<int>(H.nprocs);
-  const uint32_t wib = threadIdx.x >> 6;
-  const uint32_t nwb = blockDim.x >> 6;
-  uint32_t *rows = smem + xoff + wib * (N * 64u);
-  uint64_t *stage = reinterpret_cast<uint64_t *>(smem + ((xoff + nwb * N * 64u + 3u) & ~3u)) + wib * (kFusedStage * W);
-  __syncthreads();
-  if (C->stop) return;
-  const uint32_t cur = C->cur, nxt = cur ^ 1u;
-  const uint64_t nwork = C->front_count[cur];
-  const uint64_t *front = a.front[cur];
-  uint64_t *out = a.front[nxt];
-  unsigned long long *out_count = reinterpret_cast<unsigned long long *>(&C->front_count[nxt]);
-  constexpr int LPB = BW / 2;
-  constexpr int CPW = 64 / LPB;
-  const int lane = lane_id();
-  const int sub = lane % LPB;
-  const int cg = lane / LPB;
-  const int gbase = cg * LPB;
-  const bool safety = a.property == 2;
-  const uint64_t gw = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
-  uint64_t *slice = a.cand + gw * a.cw * W;
-  SlotFmt<W> fmt;
-  make_fmt<W, false>(fmt, 0, 0);
-  uint64_t acc_trans = 0, acc_states = 0, acc_new = 0, acc_look = 0;
-  bool viol = false, bad = false, full = false, timeout = false, overflow = false, fatal = false;
-  uint32_t staged = 0;  // wave-uniform
-  auto flush = [&]() {
-    if (!staged) return;
-    wave_sync();
-    uint64_t ob = 0;
-    if (lane == 0) ob = atomicAdd(out_count, static_cast<unsigned long long>(staged));
-    ob = __shfl(ob, 0, 64);
-    for (uint32_t i = lane; i < staged * W; i += 64) {
-      if (ob + i / W < a.front_cap)
-        out[ob * W + i] = stage[i];
-      else
-        overflow = true;
-    }
-    wave_sync();
-    staged = 0;
-  };
-  for (uint64_t b0 = gw * 64; b0 < nwork && !fatal; b0 += nw * 64) {
-    const uint64_t si = b0 + lane;
-    const bool has = si < nwork;
-    uint64_t key[W];
-#pragma unroll
-    for (int j = 0; j < W; j++) key[j] = has ? front[si * W + j] : kFlag;
-    const TpsInfo inf = tps_scan<W>(M, H, key, has, rows, lane, safety);
-    viol |= inf.viol;
-    bad |= inf.bad;
-    if (a.property == 1 && inf.dead) viol = true;  // cu:1134-1151
-    uint64_t total = 0;
-    const uint64_t off = wave_excl_scan(inf.cnt, &total);
-    acc_trans += total;
-    acc_states += __popcll(ballot(has));
-    for (uint64_t base = 0; base < total;) {
-      // whole lanes whose successors fit the window [base, base + cw)
-      const bool mine = inf.cnt > 0 && off >= base && off + inf.cnt <= base + a.cw;
-      const uint64_t mm = ballot(mine);
-      if (!mm) {  // one state has more successors than a slice holds
-        fatal = true;
-        break;
-      }
-      const int last = 63 - __clzll(static_cast<long long>(mm));
-      const uint64_t wend = __shfl(off + inf.cnt, last, 64);
-      if (mine) {
-        uint64_t pos = off - base;
-        tps_emit<W>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
-#pragma unroll
-          for (int j = 0; j < W; j++) slice[pos * W + j] = sv[j];
-          pos++;
-        });
-      }
-      __threadfence_block();  // the slice is read back by other lanes of this wave
-      wave_sync();
-      const uint64_t n = wend - base;
-      for (uint64_t c0 = 0; c0 < n; c0 += CPW) {
-        const uint64_t ci = c0 + cg;
-        const bool active = ci < n;
-        uint64_t ck[W];
-#pragma unroll
-        for (int j = 0; j < W; j++) ck[j] = active ? slice[ci * W + j] : kFlag;
-        const int st = find_or_put_group<W, BW>(a.table, ck, fmt, key_hash<W>(ck, a.table.seed), sub, gbase, active);
-        acc_look += (active && sub == 0) ? 1 : 0;
-        const bool is_new = active && st == kInserted;
-        full |= active && st == kFull;
-        timeout |= active && st == kTimeout;
-        const uint64_t m = ballot(is_new && sub == 0);
-        if (!m) continue;
-        const uint32_t cnt = __popcll(m);
-        acc_new += cnt;
-        if (staged + cnt > static_cast<uint32_t>(kFusedStage)) flush();
-        if (is_new) {
-          const uint32_t p = staged + __popcll(m & ((1ull << gbase) - 1ull));
-#pragma unroll
-          for (int j = 0; j < W; j++)
-            if (j % LPB == sub) stage[p * W + j] = ck[j];
-        }
-        staged += cnt;
-      }
-      wave_sync();  // the slice is rewritten by the next window
-      base = wend;
-    }
-  }
-  flush();
-  uint64_t looks = 0;  // lookups were counted per lane
-  (void)wave_excl_scan(acc_look, &looks);
-  if (lane == 0) {
-    add_counter(C, kCntLookups, looks);
-    add_counter(C, kCntInserts, acc_new);
-    add_counter(C, kCntTransitions, acc_trans);
-    add_counter(C, kCntExpanded, acc_states);
-  }
-  if (ballot(viol) && lane == 0) C->violation = 1;
-  if (ballot(bad) && lane == 0) C->bad_state = 1;
-  if (ballot(full) && lane == 0) C->table_full = 1;
-  if (ballot(timeout) && lane == 0) C->spin_timeout = 1;
-  if (ballot(overflow || fatal) && lane == 0) C->overflow = 1;
-}
-
-template <int W>
-static hipError_t fused_bw(const FusedArgs &a, int BW, bool lds, uint32_t blocks, uint32_t threads, hipStream_t s) {
-  const size_t shm = fused_scratch_bytes(threads, a.hdr.nprocs, W) +
-                     (lds ? (a.hdr.total_words - a.hdr.tps_begin + 3u) / 4u * 16u : 0u);
-#define GX_FB(BWk)                                                                                        \
-  if (lds) hipLaunchKernelGGL((k_fused<W, BWk, true>), dim3(blocks), dim3(threads), shm, s, a);           \
-  else hipLaunchKernelGGL((k_fused<W, BWk, false>), dim3(blocks), dim3(threads), shm, s, a);              \
-  break;
-  switch (BW) {
-    case 8: GX_FB(8)
-    case 16: GX_FB(16)
-    case 32: GX_FB(32)
-    default: return hipErrorInvalidValue;
-  }
-#undef GX_FB
-  return hipGetLastError();
-}
-
-hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, bool lds, uint32_t blocks,
-                        uint32_t threads, hipStream_t s) {
-  if (threads != 256) return hipErrorInvalidValue;
-  switch (key_words) {
-    case 1: return fused_bw<1>(a, bucket_words, lds, blocks, threads, s);
-    case 2: return fused_bw<2>(a, bucket_words, lds, blocks, threads, s);
-    case 3: return fused_bw<3>(a, bucket_words, lds, blocks, threads, s);
-    case 4: return fused_bw<4>(a, bucket_words, lds, blocks, threads, s);
-    case 5: return fused_bw<5>(a, bucket_words, lds, blocks, threads, s);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // FIND-OR-PUT over the candidate buffer.  New keys are staged per wave in
 // LDS and appended to the next frontier in blocks of up to kStage keys, one
 // atomicAdd per block: a single contended counter would otherwise serialise
@@ -383,9 +208,16 @@ hipError_t launch_fused(const FusedArgs &a, int key_words, int bucket_words, boo
 // Eight resident waves per SIMD are asked for where slots allow it
 // (kInsertMinWaves).
 
-template <int W, int WT, int BW, bool STAMP>
+// OPT: kInsDedup (the duplicate filter below) and kInsSetFlag (keys from a
+// caller: bit 63 of every word is set here) are compile-time options.  The
+// default instantiation carries neither: as runtime branches they cost the
+// plain FIND-OR-PUT registers (five VGPRs spilled to scratch) and 5-10 % of
+// its time.
+template <int W, int WT, int BW, bool STAMP, int OPT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<WT, STAMP>)))
 void k_insert(InsertArgs a) {
+  constexpr bool DEDUP = (OPT & kInsDedup) != 0;
+  constexpr uint64_t fl = (OPT & kInsSetFlag) ? kFlag : 0ull;
   extern __shared__ uint64_t stage_all[];
   Ctrl *C = a.ctrl;
   if (C->stop) return;
@@ -466,7 +298,6 @@ void k_insert(InsertArgs a) {
         r0 += nw * RUN;
         c0 = r0;
       }
-      const uint64_t fl = a.set_flag ? kFlag : 0ull;
       uint64_t key[W];
 #pragma unroll
       for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
@@ -484,7 +315,7 @@ void k_insert(InsertArgs a) {
     // hoisted out of the probe loop and computed for every candidate).
     static_assert(kInsertRun % LPB == 0, "a run is a whole number of steps");
     const uint64_t RUN = CPW * kInsertRun;
-    // Duplicate filter (a.dedup): the successors of commuting moves repeat
+    // Duplicate filter (kInsDedup): the successors of commuting moves repeat
     // within a few dozen candidates, so a candidate equal to another of the
     // same step, or of the wave's previous step, is not looked up -- the
     // other lane's lookup (this step) or the finished one (last step) stands
@@ -496,7 +327,7 @@ void k_insert(InsertArgs a) {
     uint64_t key_prev[W];
 #pragma unroll
     for (int j = 0; j < W; j++) key_prev[j] = 0;
-    if (a.dedup) {
+    if (DEDUP) {
       for (uint32_t i = lane; i < kDedupSlots; i += 64) dtag[i] = 0;
       wave_sync();
     }
@@ -508,14 +339,13 @@ void k_insert(InsertArgs a) {
         r0 += nw * RUN;
         c0 = r0;
       }
-      const uint64_t fl = a.set_flag ? kFlag : 0ull;
       uint64_t key_l[W];
 #pragma unroll
       for (int j = 0; j < W; j++) key_l[j] = cl < n ? (__builtin_nontemporal_load(a.cand + cl * W + j) | fl) : 0;
       uint64_t act_m = ballot(key_l[0] != 0);
       const uint64_t h_l = key_hash<W>(key_l, a.table.seed);
       const uint64_t b_l = bucket_of(h_l, 0, a.table.nbuckets);
-      if (a.dedup) {
+      if (DEDUP) {
         const bool act = (act_m >> lane) & 1ull;
         const uint32_t slot = static_cast<uint32_t>(h_l) & (kDedupSlots - 1u);
         auto equal_to = [&](const uint64_t (&other)[W], int src) {
@@ -726,18 +556,30 @@ hipError_t launch_expand(const ExpandArgs &a, int mode, int key_words, int slot_
   }
 }
 
-template <int W, int WT, bool STAMP>
-static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
-  const size_t shm = static_cast<size_t>(threads / 64) * (kStage * W * 8 + kDedupSlots * 4);
+template <int W, int WT, bool STAMP, int OPT>
+static hipError_t insert_bw_opt(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  const size_t shm = static_cast<size_t>(threads / 64) * (kStage * W * 8 + ((OPT & kInsDedup) ? kDedupSlots * 4 : 0));
   switch (bw) {
     case 8:
-      if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break; }
+      if (WT <= 8) { hipLaunchKernelGGL((k_insert<W, WT, 8, STAMP, OPT>), dim3(blocks), dim3(threads), shm, s, a); break; }
       return hipErrorInvalidValue;
-    case 16: hipLaunchKernelGGL((k_insert<W, WT, 16, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break;
-    case 32: hipLaunchKernelGGL((k_insert<W, WT, 32, STAMP>), dim3(blocks), dim3(threads), shm, s, a); break;
+    case 16: hipLaunchKernelGGL((k_insert<W, WT, 16, STAMP, OPT>), dim3(blocks), dim3(threads), shm, s, a); break;
+    case 32: hipLaunchKernelGGL((k_insert<W, WT, 32, STAMP, OPT>), dim3(blocks), dim3(threads), shm, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// the filter runs in explorations (every mode); caller keys (set_flag) come
+// only from the visited-set API, whose slots are the keys themselves
+template <int W, int WT, bool STAMP>
+static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
+  if (a.set_flag) {
+    if constexpr (W == WT && !STAMP) return insert_bw_opt<W, WT, STAMP, kInsSetFlag>(a, bw, blocks, threads, s);
+    return hipErrorInvalidValue;
+  }
+  if (a.dedup) return insert_bw_opt<W, WT, STAMP, kInsDedup>(a, bw, blocks, threads, s);
+  return insert_bw_opt<W, WT, STAMP, 0>(a, bw, blocks, threads, s);
 }
 
 hipError_t launch_insert(const InsertArgs &a, int key_words, int slot_words, int bucket_words,

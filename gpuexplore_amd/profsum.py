@@ -141,7 +141,7 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
                         "write_ratio": (t["write_bytes"] / n / kw) if kw else None,
                         "fetch_size_expr_ratio": t["fetch_size_expr_bytes"] / n / kr,
                         "req_64b_per_dispatch": t["req_64b"] / n, "req_128b_per_dispatch": t["req_128b"] / n}
-        if "k_expand" in k or "k_insert" in k or "k_fused" in k:
+        if "k_expand" in k or "k_insert" in k:
             n_level = sum(1 for r in trace if _short(r["Kernel_Name"]) == k and grid(r) > 64) or n
             kern[k] = {"dispatches": n, "level_launches": n_level,
                        "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"],

@@ -92,24 +92,6 @@ def test_small_models_group_kernel(gpu, oracle_mod, model_dir, name, net):
         assert same(a, b), (prop, a, b)
 
 
-@pytest.mark.parametrize("name,net", small_models(), ids=[n for n, _ in small_models()])
-def test_small_models_fused_kernel(gpu, oracle_mod, model_dir, name, net):
-    """plain mode through the fused level kernel (successors and FIND-OR-PUT
-    of a batch by the same wave); cand_capacity 2^15 over 32 workgroups gives
-    slices of 256 entries per wave, so larger batches are split into windows"""
-    p = os.path.join(model_dir, name)
-    net.write(p)
-    for prop in (0, 1, 2):
-        for kw in ({}, {"cand_capacity": 1 << 15, "blocks": 32}):
-            say(name, prop, "fused", kw)
-            a = oracle_mod.explore(p, prop)
-            b = gpu_run(gpu, p, prop, expand_kernel=3, **kw)
-            if b.get("status", 0) == -10:  # a state with more successors than a slice
-                assert kw
-                continue
-            assert same(a, b), (prop, kw, a, b)
-
-
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_config1(gpu, oracle_mod, model_dir, seed):
     p = os.path.join(model_dir, "c1")
@@ -224,7 +206,7 @@ def test_empty_sync_entry(gpu, oracle_mod, model_dir):
     for prop in (0, 1, 2):
         for por, prov in MODES:
             a = oracle_mod.explore(p, prop, por, prov)
-            kernels = (0, 1, 3) if not por else (0,)
+            kernels = (0, 1) if not por else (0,)
             for ek in kernels:
                 say("quirk empty entry", prop, por, prov, ek)
                 b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
@@ -257,13 +239,9 @@ def test_edge_models(gpu, oracle_mod, model_dir, name):
     for prop in (0, 1, 2):
         for por, prov in MODES:
             a = oracle_mod.explore(p, prop, por, prov)
-            for ek in ((0, 1, 3) if not por else (0,)):
+            for ek in ((0, 1) if not por else (0,)):
                 say("edge", name, prop, por, prov, ek)
-                try:
-                    b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
-                except gpu.GxError as e:
-                    assert ek == 3 and e.status == -7  # GX_E_ARG: no state-per-lane kernel for > 64 labels
-                    continue
+                b = gpu_run(gpu, p, prop, por, prov, expand_kernel=ek)
                 assert same(a, b), (name, prop, por, prov, ek, a, b)
 
 

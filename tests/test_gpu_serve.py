@@ -190,3 +190,16 @@ def test_serve_chunk_beyond_candidate_buffer(gpu, jobs, tmp_path):
             break
     else:
         raise AssertionError("server stuck after a candidate overflow")
+
+
+def test_serve_workload_short_stream(gpu):
+    """the benchmark's serving leg end to end on a short stream (ten jobs, two
+    of them 20-process 3e8-state ones): every served verdict and count equals
+    the job's plain-path result, in the stream and with every job submitted
+    at once"""
+    from gpuexplore_amd import serve
+    out = serve.serve_workload(20.0, 0.5, seed=0, log=say)
+    say("short stream", out["jobs"], out["states"], out["p50_time_to_verdict_s"])
+    assert out["jobs"] == 10
+    assert out["verdict_mismatches"] == 0
+    assert out["p50_time_to_verdict_s"] > 0
