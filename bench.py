@@ -35,10 +35,113 @@ GOLDEN = os.path.join(ROOT, "tests", "golden", "config2_counts.json")
 CPU_CORES = 16  # the host share of one GPU
 
 
-def batched_vs_plain(gx, gen, tmp: str) -> dict:
-    """64 models of uneven size (gen.config4_models, seeds 1000-1063; half
-    -d, half -p): one gx_explore_batch with autotuned settings against one
-    exploration per model (explorers created before timing)."""
+def insert_roofline(stats: dict, n_inst: int, opts: dict, prof_kernel: str = None, prof_file: str = None) -> dict:
+    """The batched FIND-OR-PUT's roofline from a time_kernels run: algorithmic
+    bytes (128 B per lookup and per insert, inserts = states found by the
+    kernel, i.e. without the instances' initial states) over its event-timed
+    launches; bytes to and from memory per launch from a committed counter
+    summary of the same workload run with the same options, when there is
+    one."""
+    inserts = stats["states"] - n_inst
+    algo = BYTES_PER_FINDORPUT * (stats["lookups"] + inserts)
+    t = stats["t_insert_s"]
+    out = {"kernel": "k_batch_insert (FIND-OR-PUT)", "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBPS,
+           "achieved": algo / t / 1e9, "frac": algo / t / 1e9 / PEAK_HBM_GBPS,
+           "launches": stats["insert_launches"], "avg_launch_us": 1e6 * t / max(1, stats["insert_launches"]),
+           "algorithmic_bytes_per_launch": algo / max(1, stats["insert_launches"]),
+           "t_insert_s": t, "t_expand_s": stats["t_expand_s"], "lookups": stats["lookups"],
+           "lookups_per_s": stats["lookups"] / t, "physical_gbps": None, "physical_frac": None, "traffic": None}
+    path = os.path.join(ROOT, "profiles", prof_file) if prof_file else None
+    if path and os.path.exists(path) and json.load(open(path)).get("options") == opts:
+        pj = json.load(open(path))
+        for k, v in pj["kernels"].items():
+            if prof_kernel and prof_kernel in k:
+                out["traffic"] = v["bytes_per_launch"]
+                out["physical_gbps"] = v["bytes_per_launch"] / (t / max(1, stats["insert_launches"])) / 1e9
+                out["physical_frac"] = out["physical_gbps"] / PEAK_HBM_GBPS
+                out["traffic_source"] = "profiles/" + prof_file
+    return out
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def plain_runs(gx, models, props, plain_q: int, **kw) -> list:
+    """Each model alone through a gx_explorer (created outside the timing,
+    one at a time): the second of two runs, whose counts must equal the
+    first's."""
+    out = []
+    for i, (m, pr) in enumerate(zip(models, props)):
+        ex = gx.Explorer(m, property=pr, q_ints=plain_q, **kw)
+        r0, r = ex.run(), ex.run()
+        if r["table_full"] or (r["states"], r["transitions"]) != (r0["states"], r0["transitions"]):
+            raise RuntimeError(f"plain runs of model {i} differ or filled the table")
+        out.append(r)
+        del ex
+        if (i + 1) % 16 == 0:
+            log(f"plain path: {i + 1} of {len(models)} models")
+    return out
+
+
+def batch_leg(gx, models, props, q_batch: int, opts: dict, plain: list, weights=None,
+              prof_file: str = None) -> dict:
+    """One batch of models through gx_explore_batch (best of three, with the
+    given options) and once more with kernel timing (rounds launched one by
+    one, HIP events around the FIND-OR-PUT launches), against the models'
+    plain runs (plain_runs).  Counts must be equal."""
+    runs = []
+    for _ in range(3):
+        res, stats, rc = gx.explore_batch(models, props, weights=weights, q_ints=q_batch, **opts)
+        if rc != 0 or any(r["table_full"] for r in res):
+            raise RuntimeError(f"batch failed (status {rc}) or a partition filled up")
+        runs.append(stats)
+    t_batch = min(s["t_explore_s"] for s in runs)
+    states = runs[0]["states"]
+    tk = {k: v for k, v in opts.items() if k != "use_graph"}
+    tres, tstats, rc = gx.explore_batch(models, props, weights=weights, q_ints=q_batch, time_kernels=1, **tk)
+    if rc != 0 or [(r["states"], r["transitions"]) for r in tres] != [(r["states"], r["transitions"]) for r in res]:
+        raise RuntimeError("the kernel-timed batch run differs")
+    for i, r in enumerate(plain):
+        if (r["states"], r["transitions"]) != (res[i]["states"], res[i]["transitions"]):
+            raise RuntimeError(f"batched and plain counts differ for model {i}")
+    t_plain = sum(r["t_explore_s"] for r in plain)
+    log(f"batch: {states} states, batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms")
+    return {
+        "states": states,
+        "transitions": runs[0]["transitions"],
+        "batched_new_states_per_s": states / t_batch,
+        "plain_new_states_per_s": states / t_plain,
+        "ratio": t_plain / t_batch,
+        "t_batched_s": t_batch,
+        "t_plain_s": t_plain,
+        "options": opts,
+        "rounds": runs[0]["rounds"],
+        "graph_launches": runs[0]["graph_launches"],
+        "batch_insert_roofline": insert_roofline(tstats, len(models), opts, "k_batch_insert", prof_file),
+    }
+
+
+def batched_vs_plain(gx, gen, tmp: str, legs: str = "both", tune: bool = True) -> dict:
+    """config 4 (64 models of uneven size, gen.config4_models seeds 1000-1063,
+    half -d and half -p; options autotuned) and a batch at the large plain
+    workload's scale (gen.config2_batch: 64 config-2-generator models of
+    14-17 processes, about 1e9 states together, no property; partitions
+    sized from the plain runs' counts) against one exploration per model.
+    legs: "both", "config4" or "large"; tune=False takes the library's
+    default options for config 4 and 16384 FIND-OR-PUT workgroups for the
+    large batch (for profiling runs)."""
+    from gpuexplore_amd import autotune
+
+    out = {}
+    if legs in ("both", "config4"):
+        out.update(config4_leg(gx, gen, tmp, tune))
+    if legs in ("both", "large"):
+        out["large"] = large_leg(gx, gen, tmp, tune)
+    return out
+
+
+def config4_leg(gx, gen, tmp: str, tune: bool) -> dict:
     from gpuexplore_amd import autotune
 
     paths, props = [], []
@@ -49,39 +152,44 @@ def batched_vs_plain(gx, gen, tmp: str) -> dict:
         props.append(gx.PROP_DEADLOCK if flag == "-d" else gx.PROP_SAFETY)
     models = [gx.Model.load(p) for p in paths]
     q_batch = 64 << 24  # 64 MiB of visited set per instance
-    tuned = autotune.tune_batch(models, props, q_batch, repeats=2)
-    best = tuned["best"]
-    runs = []
-    for _ in range(3):
-        res, stats, rc = gx.explore_batch(models, props, q_ints=q_batch, **best)
-        runs.append(stats)
-    t_batch = min(s["t_explore_s"] for s in runs)
-    states = runs[0]["states"]
-    if any(r["table_full"] for r in res):
-        raise RuntimeError("a batch partition filled up")
-    explorers = [gx.Explorer(m, property=pr, q_ints=1 << 24) for m, pr in zip(models, props)]
-    plain_t = []
-    for _ in range(2):
-        t = 0.0
-        for i, ex in enumerate(explorers):
-            r = ex.run()
-            if r["table_full"] or (r["states"], r["transitions"]) != (res[i]["states"], res[i]["transitions"]):
-                raise RuntimeError(f"batched and plain counts differ for model {i}")
-            t += r["t_explore_s"]
-        plain_t.append(t)
-    t_plain = min(plain_t)
-    return {
-        "workload": "config4: 64 models, N 4..14, 4..32 states, 32 -d + 32 -p",
-        "states": states,
-        "batched_new_states_per_s": states / t_batch,
-        "plain_new_states_per_s": states / t_plain,
-        "ratio": t_plain / t_batch,
-        "t_batched_s": t_batch,
-        "t_plain_s": t_plain,
-        "autotuned": best,
-        "rounds": runs[0]["rounds"],
-        "graph_launches": runs[0]["graph_launches"],
-    }
+    log("config 4 batch: autotuning" if tune else "config 4 batch")
+    best = autotune.tune_batch(models, props, q_batch, repeats=2)["best"] if tune else {"levels_per_sync": 16}
+    plain = plain_runs(gx, models, props, 1 << 24)
+    out = {"workload": "config4: 64 models, N 4..14, 4..32 states, 32 -d + 32 -p", "autotuned": tune}
+    out.update(batch_leg(gx, models, props, q_batch, best, plain,
+                         prof_file="k_batch_insert_config4_traffic.json"))
+    return out
+
+
+def large_leg(gx, gen, tmp: str, tune: bool) -> dict:
+    from gpuexplore_amd import autotune
+
+    nets = gen.config2_batch()
+    lm = []
+    for i, net in enumerate(nets):
+        p = os.path.join(tmp, f"c2b_{i}")
+        net.write(p)
+        lm.append(gx.Model.load(p))
+    lprops = [gx.PROP_NONE] * len(lm)
+    log("large batch: plain path")
+    # plain explorers of a 2 GiB visited set each, with bounded frontier and
+    # successor buffers (a larger level is deferred, as in any exploration)
+    plain = plain_runs(gx, lm, lprops, 1 << 29, frontier_capacity=1 << 26, cand_capacity=1 << 28)
+    # batch partitions in proportion to each model's state count, at most half full
+    counts = [r["states"] for r in plain]
+    q_large = max(1 << 28, 2 * 4 * sum(counts))  # 16-byte slots (4 ints), load <= 1/2
+    log("large batch: autotuning" if tune else "large batch")
+    if tune:
+        best = autotune.tune_batch(lm, lprops, q_large, space={"insert_blocks": [0, 16384], "levels_per_sync": [16]},
+                                   repeats=1, weights=counts)["best"]
+    else:
+        best = {"insert_blocks": 16384, "levels_per_sync": 16}
+    leg = batch_leg(gx, lm, lprops, q_large, best, plain, weights=counts,
+                    prof_file="k_batch_insert_config2batch_traffic.json")
+    leg["workload"] = ("64 config-2-generator models (gen.config2_batch: 14-17 processes x 16 states, "
+                       "token budgets, no property)")
+    leg["table_bytes"] = 4 * q_large
+    return leg
 
 
 def por_reduction(gx, gen, tmp: str) -> dict:
@@ -129,7 +237,11 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=0, help="successor-kernel workgroup size, 0 = library default")
     ap.add_argument("--insert-blocks", type=int, default=0, help="FIND-OR-PUT workgroups per launch, 0 = sized per level")
     ap.add_argument("--dedup", type=int, default=0, help="FIND-OR-PUT duplicate filter (1 on, 0 off)")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs plain comparisons")
+    ap.add_argument("--batch", choices=("both", "config4", "large"), default="both",
+                    help="batched legs: config 4 and/or 64 config-2-generator models (large)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched legs with fixed options (profiling runs)")
     ap.add_argument("--no-por", action="store_true", help="skip the config 3 partial-order reduction runs")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson job stream through the server (0 skips it)")
@@ -211,11 +323,9 @@ def main() -> int:
                 physical = v.get("dram_bytes_per_launch")
                 atomics = v.get("atomics_per_launch")
 
-    batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp)
-    por = None
-    if not args.no_por:
-        del ex  # the config 2 table (2^34 ints) is not needed any more
-        por = por_reduction(gx, gen, tmp)
+    del ex  # the config 2 buffers (a 2^34-int visited set) are not needed any more
+    batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp, args.batch, not args.no_autotune)
+    por = None if args.no_por else por_reduction(gx, gen, tmp)
 
     serving = None
     if args.serve_seconds > 0:

@@ -89,6 +89,7 @@ class BatchOptions(ctypes.Structure):
         ("cand_capacity", ctypes.c_uint64),
         ("use_graph", ctypes.c_uint32),
         ("insert_blocks", ctypes.c_uint32),
+        ("time_kernels", ctypes.c_uint32),
     ]
 
 
@@ -102,6 +103,9 @@ class BatchStats(ctypes.Structure):
         ("states", ctypes.c_uint64),
         ("transitions", ctypes.c_uint64),
         ("cand_entries", ctypes.c_uint64),
+        ("t_expand_s", ctypes.c_double),
+        ("t_insert_s", ctypes.c_double),
+        ("insert_launches", ctypes.c_uint64),
     ]
 
 

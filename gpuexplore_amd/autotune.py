@@ -31,9 +31,10 @@ def _signature(res: List[dict]) -> tuple:
 
 def tune_batch(models: Sequence["gx.Model"], properties: Sequence[int], q_ints: int,
                space: Optional[Dict[str, list]] = None, repeats: int = 2,
-               cache_path: Optional[str] = None, key: Optional[str] = None) -> dict:
+               cache_path: Optional[str] = None, key: Optional[str] = None, weights=None) -> dict:
     """Returns {"best": {option: value}, "t_explore_s": t, "table": [...]}.
-    With cache_path and key, a previous result for the same key is reused."""
+    With cache_path and key, a previous result for the same key is reused.
+    ``weights`` sizes the instances' visited-set partitions (gx_explore_batch)."""
     if cache_path and key and os.path.exists(cache_path):
         cached = json.load(open(cache_path))
         if key in cached:
@@ -46,7 +47,8 @@ def tune_batch(models: Sequence["gx.Model"], properties: Sequence[int], q_ints: 
         opts = dict(zip(names, values))
         times = []
         for _ in range(repeats):
-            res, stats, rc = gx.explore_batch(list(models), list(properties), q_ints=q_ints, **opts)
+            res, stats, rc = gx.explore_batch(list(models), list(properties), weights=weights, q_ints=q_ints,
+                                              **opts)
             if rc != 0:
                 raise RuntimeError(f"batch failed with {opts}: {rc}")
             sig = _signature(res)

@@ -79,8 +79,14 @@ inline uint32_t batch_mode(int por, int cycle_proviso) {
 
 // expand (plain, and POR when a.por) + FIND-OR-PUT + advance of every
 // unfinished instance (one BFS level)
+// Events recorded before the successor launches, between them and the
+// FIND-OR-PUT launch, and after it (kernel timing of a batch round).
+struct RoundEvents {
+  hipEvent_t e0, e1, e2;
+};
+
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
-                          uint32_t ins_blocks, hipStream_t s);
+                          uint32_t ins_blocks, hipStream_t s, const RoundEvents *ev = nullptr);
 // LDS of the plain batch expand kernel: 40 KiB (four workgroups per CU, as
 // the single-model kernel gets on a 20-process model), or the waves' row
 // tables for the largest model when they need more.  A model is staged when

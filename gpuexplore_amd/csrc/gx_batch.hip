@@ -641,8 +641,12 @@ static void launch_insert(const BatchArgs &a, uint32_t blocks, hipStream_t s) {
 
 template <int W>
 hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t threads, uint32_t ins_blocks,
-                        hipStream_t s) {
+                        hipStream_t s, const RoundEvents *ev) {
   const size_t shm = a.expand_lds;
+  if (ev) {
+    const hipError_t e = hipEventRecord(ev->e0, s);
+    if (e != hipSuccess) return e;
+  }
   if (a.rows16)
     hipLaunchKernelGGL((k_batch_expand<W, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
   else
@@ -651,24 +655,32 @@ hipError_t launch_round(const BatchArgs &a, int BW, uint32_t blocks, uint32_t th
     const size_t pshm = static_cast<size_t>(a.por_threads / 64) * kGroupScratchBytesPerWave;
     hipLaunchKernelGGL((k_batch_expand_por<W>), dim3(a.por_blocks), dim3(a.por_threads), pshm, s, a);
   }
+  if (ev) {
+    const hipError_t e = hipEventRecord(ev->e1, s);
+    if (e != hipSuccess) return e;
+  }
   switch (BW) {
     case 8: launch_insert<W, 8>(a, ins_blocks, s); break;
     case 16: launch_insert<W, 16>(a, ins_blocks, s); break;
     case 32: launch_insert<W, 32>(a, ins_blocks, s); break;
     default: return hipErrorInvalidValue;
   }
+  if (ev) {
+    const hipError_t e = hipEventRecord(ev->e2, s);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1024), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_round_w(const BatchArgs &a, int W, int BW, uint32_t blocks, uint32_t threads,
-                          uint32_t ins_blocks, hipStream_t s) {
+                          uint32_t ins_blocks, hipStream_t s, const RoundEvents *ev) {
   switch (W) {
-    case 1: return launch_round<1>(a, BW, blocks, threads, ins_blocks, s);
-    case 2: return launch_round<2>(a, BW, blocks, threads, ins_blocks, s);
-    case 3: return launch_round<3>(a, BW, blocks, threads, ins_blocks, s);
-    case 4: return launch_round<4>(a, BW, blocks, threads, ins_blocks, s);
-    case 5: return launch_round<5>(a, BW, blocks, threads, ins_blocks, s);
+    case 1: return launch_round<1>(a, BW, blocks, threads, ins_blocks, s, ev);
+    case 2: return launch_round<2>(a, BW, blocks, threads, ins_blocks, s, ev);
+    case 3: return launch_round<3>(a, BW, blocks, threads, ins_blocks, s, ev);
+    case 4: return launch_round<4>(a, BW, blocks, threads, ins_blocks, s, ev);
+    case 5: return launch_round<5>(a, BW, blocks, threads, ins_blocks, s, ev);
     default: return hipErrorInvalidValue;
   }
 }
@@ -925,11 +937,27 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
   BX_TRY(hipStreamSynchronize(st));
   const double t_init = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_init0).count();
 
-  const uint32_t k = std::max<uint32_t>(1, o->levels_per_sync);
+  // kernel timing: one round per host synchronisation, launched directly
+  const bool timed = o->time_kernels != 0;
+  const uint32_t k = timed ? 1u : std::max<uint32_t>(1, o->levels_per_sync);
   BatchCtrl ch{};
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
-  if (o->use_graph) {
+  struct EventGuard {
+    RoundEvents ev{nullptr, nullptr, nullptr};
+    ~EventGuard() {
+      for (hipEvent_t e : {ev.e0, ev.e1, ev.e2})
+        if (e) (void)hipEventDestroy(e);
+    }
+  } eg;
+  if (timed) {
+    BX_TRY(hipEventCreate(&eg.ev.e0));
+    BX_TRY(hipEventCreate(&eg.ev.e1));
+    BX_TRY(hipEventCreate(&eg.ev.e2));
+  }
+  double t_exp_ms = 0, t_ins_ms = 0;
+  uint64_t ins_launches = 0;
+  if (o->use_graph && !timed) {
     BX_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t ce = hipSuccess;
     for (uint32_t r = 0; r < k && ce == hipSuccess; r++)
@@ -953,11 +981,20 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
     if (exec) {
       BX_TRY(hipGraphLaunch(exec, st));
     } else {
-      for (uint32_t r = 0; r < k; r++) BX_TRY(launch_round_w(a, W, BW, blocks, threads, ins_blocks, st));
+      for (uint32_t r = 0; r < k; r++)
+        BX_TRY(launch_round_w(a, W, BW, blocks, threads, ins_blocks, st, timed ? &eg.ev : nullptr));
     }
     launches++;
     BX_TRY(hipMemcpyAsync(&ch, d_ctrl.p, sizeof ch, hipMemcpyDeviceToHost, st));
     BX_TRY(hipStreamSynchronize(st));
+    if (timed) {
+      float ms = 0;
+      BX_TRY(hipEventElapsedTime(&ms, eg.ev.e0, eg.ev.e1));
+      t_exp_ms += ms;
+      BX_TRY(hipEventElapsedTime(&ms, eg.ev.e1, eg.ev.e2));
+      t_ins_ms += ms;
+      ins_launches++;
+    }
     if (ch.active == 0) break;
   }
   const double t_explore = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -985,6 +1022,9 @@ extern "C" int gx_explore_batch_jobs(const gx_model *const *models, const gx_job
     stats->graph_launches = launches;
     stats->lookups = ch.lookups;
     stats->cand_entries = ch.cand_entries;
+    stats->t_expand_s = t_exp_ms * 1e-3;
+    stats->t_insert_s = t_ins_ms * 1e-3;
+    stats->insert_launches = ins_launches;
     for (int i = 0; i < n; i++) {
       stats->states += inst[i].inserts;
       stats->transitions += inst[i].transitions;

@@ -441,6 +441,22 @@ def config4_models(seed0: int = 1000, count: int = 64) -> List[Tuple[Network, st
     return out
 
 
+def config2_batch(seed0: int = 2000, count: int = 64, nmin: int = 14, nmax: int = 17) -> List[Network]:
+    """A batch at the large plain workload's scale: ``count`` models from the
+    config-2 generator (16 local states = 4 control states x 4 tokens, token
+    budgets of 2 for every fifth process and 1 otherwise, out-degree 3: a
+    local move, a neighbour rendezvous and an n-way barrier) with n uniform
+    in nmin..nmax processes.  At 14..17 processes a model has 1e6..5e7
+    states and the batch about 1e9; it is explored without a property."""
+    out = []
+    for i in range(count):
+        rng = Rng(0xB16 * 1000003 + seed0 + i)
+        n = nmin + rng.below(nmax - nmin + 1)
+        budgets = [2 if p % 5 == 0 else 1 for p in range(n)]
+        out.append(rounds(n, 16, 4, budgets, 4, seed0 + i))
+    return out
+
+
 def config5_job(seed: int) -> Tuple[Network, str, int, bool]:
     """One job of the serving stream: 90% drawn from the config-3 generator
     (24 processes, 8 local states, phased rounds) and checked with

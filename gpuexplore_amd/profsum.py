@@ -109,19 +109,35 @@ def traffic(rd: Dict[str, float], wr: Dict[str, float]) -> Dict[str, float]:
             "req_128b": rd["TCC_EA0_RDREQ_128B_sum"], "fetch_size_expr_bytes": fetch_size}
 
 
-def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
+def _get(obj: dict, path: str):
+    for k in path.split("."):
+        obj = obj[k]
+    return obj
+
+
+def summarize(d: str, name: str, kernel: str, out_dir: str, roof_path: str = "roofline",
+              cmd: str = BENCH_CMD) -> Dict:
+    """Summary of one collected directory for `kernel`: its kernel-trace
+    average, its bytes to and from memory, and the algorithmic bytes per
+    launch from the bench line's roofline object at `roof_path` (the
+    single-model FIND-OR-PUT's by default; a batch leg's
+    batch_insert_roofline for the batched one)."""
     bench = None
     for ln in open(os.path.join(d, "kt.stdout")).read().splitlines():
         if ln.startswith("{"):
             bench = json.loads(ln)
     if bench is None:
         raise RuntimeError("no bench line in the kernel-trace run")
+    roof = _get(bench, roof_path)
+    workload = bench["config"]["workload"] if roof_path == "roofline" else \
+        _get(bench, roof_path.rsplit(".", 1)[0])["workload"]
     stats = list(csv.DictReader(open(_one(os.path.join(d, "kt", "**", "*kernel_stats.csv")))))
     trace = list(csv.DictReader(open(_one(os.path.join(d, "kt", "**", "*kernel_trace.csv")))))
 
     def grid(r):
         return int(r.get("Grid_Size", r.get("Grid_Size_X", "0")) or 0)
 
+    # level launches (the single-model initial FIND-OR-PUT of s0 is one wave)
     level = [r for r in trace if kernel in r["Kernel_Name"] and grid(r) > 64]
     durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in level]
     rd, rdisp = _counters(_one(os.path.join(d, "rd", "**", "*counter_collection.csv")))
@@ -141,7 +157,7 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
                         "write_ratio": (t["write_bytes"] / n / kw) if kw else None,
                         "fetch_size_expr_ratio": t["fetch_size_expr_bytes"] / n / kr,
                         "req_64b_per_dispatch": t["req_64b"] / n, "req_128b_per_dispatch": t["req_128b"] / n}
-        if "k_expand" in k or "k_insert" in k:
+        if "expand" in k or "insert" in k:
             n_level = sum(1 for r in trace if _short(r["Kernel_Name"]) == k and grid(r) > 64) or n
             kern[k] = {"dispatches": n, "level_launches": n_level,
                        "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"],
@@ -151,29 +167,31 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
                        "read_requests_per_launch": t["read_requests"] / n_level,
                        "req_32b": t["req_32b"], "req_64b": t["req_64b"], "req_128b": t["req_128b"],
                        "read_requests_unsized": t["read_requests_unsized"]}
-    roof = bench["roofline"]
     avg_us = sum(durs) / len(durs) / 1e3 if durs else None
-    data = {"workload": bench["config"]["workload"],
-            "command": f"rocprofv3 --pmc {' '.join(RD)} | --pmc {' '.join(WR)} -- {BENCH_CMD}",
+    data = {"workload": workload,
+            "command": f"rocprofv3 --pmc {' '.join(RD)} | --pmc {' '.join(WR)} -- {cmd}",
             "accounting": "read = 32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B; "
                           "write = 64 WRREQ_64B + 32 (WRREQ - WRREQ_64B); no scale factor",
             "validation": check, "kernels": kern,
+            "options": None if roof_path == "roofline" else _get(bench, roof_path.rsplit(".", 1)[0]).get("options"),
             "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
             "trace_avg_launch_us": avg_us, "bench_avg_launch_us": roof["avg_launch_us"]}
     os.makedirs(out_dir, exist_ok=True)
     with open(os.path.join(out_dir, f"{name}_traffic.json"), "w") as f:
         json.dump(data, f, indent=1)
-    md = [f"# {kernel} on {bench['config']['workload']}", "",
+    extra = "" if kernel == "k_insert" else f" --kernel {kernel}"
+    extra += "" if roof_path == "roofline" else f" --roof {roof_path} --cmd '{cmd}'"
+    md = [f"# {kernel} on {workload}", "",
           "From the repository root (each profiler pass runs in /tmp with TMPDIR=/tmp; <dir> absolute):", "",
-          "```", f"python3 -m gpuexplore_amd.profsum collect --out <dir> -- {BENCH_CMD}",
-          f"python3 -m gpuexplore_amd.profsum summarize --dir <dir> --name {name}", "```", "",
-          "Kernel time, `rocprofv3 --kernel-trace --stats`, one exploration:", "",
+          "```", f"python3 -m gpuexplore_amd.profsum collect --out <dir> -- {cmd}",
+          f"python3 -m gpuexplore_amd.profsum summarize --dir <dir> --name {name}{extra}", "```", "",
+          "Kernel time, `rocprofv3 --kernel-trace --stats`, the whole command:", "",
           "| kernel | calls | total ms | average us |", "|---|---|---|---|"]
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:10]:
         md.append(f"| `{r['Name'][:60]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
                   f"{float(r['AverageNs']) / 1e3:.1f} |")
-    md += ["", f"{kernel} average over its {len(durs)} level launches (the 1-candidate initial launch excluded): "
-               f"{avg_us:.0f} us (kernel trace); bench.py's HIP-event average for the same launches: "
+    md += ["", f"{kernel} average over its {len(durs)} launches (a one-wave initial launch excluded): "
+               f"{avg_us:.0f} us (kernel trace); bench.py's HIP-event average: "
                f"{roof['avg_launch_us']:.0f} us.", "",
            "Bytes between L2 and memory, from two counter runs of the same command "
            f"(`--pmc {' '.join(RD)}`, `--pmc {' '.join(WR)}`): read = 32 x RDREQ_32B + 64 x RDREQ_64B + "
@@ -185,7 +203,7 @@ def summarize(d: str, name: str, kernel: str, out_dir: str) -> Dict:
         md.append(f"| `{k}` | {v['dispatches']} | {v['known_read_bytes'] / 1e9:.2f} | {v['read_bytes'] / 1e9:.2f} | "
                   f"{v['read_ratio']:.3f} | {v['known_write_bytes'] / 1e9:.2f} | {v['write_bytes'] / 1e9:.2f} | "
                   f"{v['fetch_size_expr_ratio']:.3f} |")
-    md += ["", "| kernel | level launches | read GB | write GB | GB per launch | 64 B / 128 B read requests | "
+    md += ["", "| kernel | launches | read GB | write GB | GB per launch | 64 B / 128 B read requests | "
                "atomics per launch |", "|---|---|---|---|---|---|---|"]
     for k, v in kern.items():
         md.append(f"| `{k}` | {v['level_launches']} | {v['read_bytes'] / 1e9:.1f} | {v['write_bytes'] / 1e9:.1f} | "
@@ -216,11 +234,13 @@ def main(argv=None) -> int:
     s.add_argument("--name", required=True)
     s.add_argument("--kernel", default="k_insert")
     s.add_argument("--profiles", default=os.path.join(ROOT, "profiles"))
+    s.add_argument("--roof", default="roofline", help="path of the roofline object in the bench line")
+    s.add_argument("--cmd", default=BENCH_CMD, help="the profiled command, as recorded in the summary")
     a = ap.parse_args(argv)
     if a.cmd == "collect":
         rest = a.rest[1:] if a.rest and a.rest[0] == "--" else a.rest
         return collect(os.path.abspath(a.out), rest, a.limit)
-    print(json.dumps(summarize(os.path.abspath(a.dir), a.name, a.kernel, a.profiles), indent=1))
+    print(json.dumps(summarize(os.path.abspath(a.dir), a.name, a.kernel, a.profiles, a.roof, a.cmd), indent=1))
     return 0
 
 

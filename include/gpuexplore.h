@@ -190,6 +190,8 @@ typedef struct gx_batch_options {
   uint64_t cand_capacity;    /* successor buffer entries (0 = auto) */
   uint32_t use_graph;        /* 1: replay captured rounds; 0: launch them one by one */
   uint32_t insert_blocks;    /* FIND-OR-PUT workgroups of 256 threads per round (0 = 8 per CU) */
+  uint32_t time_kernels;     /* 1: launch rounds one by one (no graph) with HIP events around the
+                                successor and FIND-OR-PUT launches (gx_batch_stats.t_*_s) */
 } gx_batch_options;
 
 typedef struct gx_batch_stats {
@@ -202,6 +204,9 @@ typedef struct gx_batch_stats {
   uint64_t transitions;      /* sum over instances */
   uint64_t cand_entries;     /* successor-buffer entries scanned by FIND-OR-PUT (successors and the
                                 released tails of reservations) */
+  double t_expand_s;         /* with time_kernels: successor kernels' time, summed over rounds */
+  double t_insert_s;         /* with time_kernels: FIND-OR-PUT kernel time, summed over rounds */
+  uint64_t insert_launches;  /* with time_kernels: FIND-OR-PUT launches timed */
 } gx_batch_stats;
 
 void gx_batch_options_default(gx_batch_options *o);

@@ -91,3 +91,16 @@ def test_batch_weights_and_full_partition(gpu, batch_models):
         if i != big and not res[i]["table_full"]:
             o = batch_models[i][2]
             assert (res[i]["states"], res[i]["transitions"]) == (o["states"], o["transitions"])
+
+
+def test_batch_kernel_timing(gpu, batch_models):
+    """time_kernels: rounds launched one by one with events around the
+    successor and FIND-OR-PUT launches; counts equal the oracle's, one timed
+    FIND-OR-PUT launch per round"""
+    say("batch time_kernels")
+    res, stats, rc = run_batch(gpu, batch_models, time_kernels=1)
+    assert rc == 0
+    check(batch_models, res)
+    assert stats["insert_launches"] == stats["rounds"] == stats["graph_launches"]
+    assert 0 < stats["t_insert_s"] < stats["t_explore_s"]
+    assert 0 < stats["t_expand_s"] < stats["t_explore_s"]
