@@ -56,9 +56,12 @@ def insert_roofline(stats: dict, n_inst: int, opts: dict, prof_kernel: str = Non
         pj = json.load(open(path))
         for k, v in pj["kernels"].items():
             if prof_kernel and prof_kernel in k:
-                out["traffic"] = v["bytes_per_launch"]
-                out["physical_gbps"] = v["bytes_per_launch"] / (t / max(1, stats["insert_launches"])) / 1e9
-                out["physical_frac"] = out["physical_gbps"] / PEAK_HBM_GBPS
+                # bytes to and from memory over the kernel's time, every
+                # launch of the profiled command (rounds that find every
+                # instance finished move nothing and take microseconds)
+                out["physical_gbps"] = v["gbps"]
+                out["physical_frac"] = v["gbps"] / PEAK_HBM_GBPS
+                out["traffic"] = v["gbps"] * 1e9 * (t / max(1, stats["insert_launches"]))
                 out["traffic_source"] = "profiles/" + prof_file
     return out
 

@@ -159,7 +159,13 @@ def summarize(d: str, name: str, kernel: str, out_dir: str, roof_path: str = "ro
                         "req_64b_per_dispatch": t["req_64b"] / n, "req_128b_per_dispatch": t["req_128b"] / n}
         if "expand" in k or "insert" in k:
             n_level = sum(1 for r in trace if _short(r["Kernel_Name"]) == k and grid(r) > 64) or n
+            t_ns = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace
+                       if _short(r["Kernel_Name"]) == k)
             kern[k] = {"dispatches": n, "level_launches": n_level,
+                       # every dispatch's bytes over every dispatch's time (the
+                       # trace run's): launches that find no work weigh nothing
+                       "gbps": (t["read_bytes"] + t["write_bytes"]) / t_ns if t_ns else None,
+                       "kernel_ms": t_ns / 1e6,
                        "read_bytes": t["read_bytes"], "write_bytes": t["write_bytes"],
                        "bytes_per_launch": (t["read_bytes"] + t["write_bytes"]) / n_level,
                        "dram_bytes_per_launch": (t["read_bytes"] + t["write_bytes"]) / n_level,
@@ -211,15 +217,47 @@ def summarize(d: str, name: str, kernel: str, out_dir: str, roof_path: str = "ro
                   f"{v['atomics_per_launch'] / 1e6:.1f} M |")
     ki = next((v for k, v in kern.items() if kernel in k), None)
     if ki and avg_us:
-        phys = ki["bytes_per_launch"] / (avg_us * 1e-6) / 1e9
+        phys = ki["gbps"]
         md += ["", f"Algorithmic bytes per {kernel} launch (128 B x (lookups + inserts)): "
                    f"{roof['algorithmic_bytes_per_launch'] / 1e9:.2f} GB, {roof['achieved']:.0f} GB/s = "
                    f"{100 * roof['frac']:.1f} % of {roof['peak']:.0f} GB/s.  Counted bytes to and from memory: "
-                   f"{ki['bytes_per_launch'] / 1e9:.2f} GB per launch, {phys:.0f} GB/s = {100 * phys / roof['peak']:.1f} % "
-                   f"of the peak; atomics {ki['atomics_per_launch'] / (avg_us * 1e-6) / 1e9:.2f} G/s.", ""]
+                   f"{ki['bytes_per_launch'] / 1e9:.2f} GB per launch; over the kernel's own time (every launch's "
+                   f"bytes over every launch's duration, {ki['kernel_ms']:.1f} ms) {phys:.0f} GB/s = "
+                   f"{100 * phys / roof['peak']:.1f} % of the peak; atomics "
+                   f"{ki['atomics_per_launch'] * ki['level_launches'] / (ki['kernel_ms'] * 1e-3) / 1e9:.2f} G/s.", ""]
     with open(os.path.join(out_dir, f"{name}.md"), "w") as f:
         f.write("\n".join(md))
     return data
+
+
+def gaps(d: str, prefix: str = "k_batch", split_ns: int = 2_000_000) -> List[Dict]:
+    """Idle time between the batch's kernels in the kernel-trace run: the
+    launches whose names start with `prefix`, split into explorations where
+    consecutive launches are more than `split_ns` apart (host work between
+    explorations).  Per exploration: kernel busy time, span from the first
+    start to the last end, and the gaps (what one persistent kernel could
+    at most save)."""
+    trace = list(csv.DictReader(open(_one(os.path.join(d, "kt", "**", "*kernel_trace.csv")))))
+    rows = sorted((r for r in trace if _base(r["Kernel_Name"]).startswith(prefix)),
+                  key=lambda r: int(r["Start_Timestamp"]))
+    runs, cur = [], []
+    for r in rows:
+        if cur and int(r["Start_Timestamp"]) - int(cur[-1]["End_Timestamp"]) > split_ns:
+            runs.append(cur)
+            cur = []
+        cur.append(r)
+    if cur:
+        runs.append(cur)
+    out = []
+    for run in runs:
+        busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in run)
+        span = int(run[-1]["End_Timestamp"]) - int(run[0]["Start_Timestamp"])
+        g = [int(b["Start_Timestamp"]) - int(a["End_Timestamp"]) for a, b in zip(run, run[1:])]
+        out.append({"launches": len(run), "busy_ms": busy / 1e6, "span_ms": span / 1e6,
+                    "idle_ms": (span - busy) / 1e6, "idle_frac": (span - busy) / span if span else 0.0,
+                    "gaps_over_20us": sum(1 for x in g if x > 20000),
+                    "median_gap_us": sorted(g)[len(g) // 2] / 1e3 if g else 0.0})
+    return out
 
 
 def main(argv=None) -> int:
@@ -236,7 +274,14 @@ def main(argv=None) -> int:
     s.add_argument("--profiles", default=os.path.join(ROOT, "profiles"))
     s.add_argument("--roof", default="roofline", help="path of the roofline object in the bench line")
     s.add_argument("--cmd", default=BENCH_CMD, help="the profiled command, as recorded in the summary")
+    g = sub.add_parser("gaps")
+    g.add_argument("--dir", required=True)
+    g.add_argument("--prefix", default="k_batch")
     a = ap.parse_args(argv)
+    if a.cmd == "gaps":
+        for r in gaps(os.path.abspath(a.dir), a.prefix):
+            print(json.dumps(r))
+        return 0
     if a.cmd == "collect":
         rest = a.rest[1:] if a.rest and a.rest[0] == "--" else a.rest
         return collect(os.path.abspath(a.out), rest, a.limit)
