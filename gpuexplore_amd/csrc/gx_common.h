@@ -53,10 +53,10 @@ struct ModelHdr {
   uint32_t off_rowlab;     // nrows x 2 words: mask of the synchronising labels of the row (nlab <= 64)
   uint32_t lab64;          // 1 when every label fits the 64-bit row masks
   // packed copies for the state-per-lane kernels (16-byte aligned offsets)
-  uint32_t off_rowrec;     // nrows x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi, 0, 0
+  uint32_t off_rowrec;     // (nrows + 1) x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi, 0, 0
   uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
-  uint32_t off_fdesc;      // per process x 8 words: field descriptor (key word, shift, value
-                           // mask, 63 - shift, 64-bit in-word mask, 64-bit mask in the next word)
+  uint32_t off_fdesc;      // per process x 4 words: key word | shift << 8, value mask, first
+                           // row, rows (row records have one more, a bad one, at nrows)
   uint32_t off_locx;       // per loc_tgt entry: the target placed in its field (key_words
                            // 64-bit words as lo, hi pairs; zero outside the field)
   uint32_t off_synx;       // per syn_tgt entry: the same for synchronising targets

@@ -79,45 +79,6 @@ __device__ __forceinline__ void kset(uint64_t (&k)[W], uint32_t floc, uint32_t v
   }
 }
 
-// The same codec through a per-process field descriptor (8 words, see
-// ModelHdr::off_fdesc): no per-call mask construction.
-struct FieldDesc {
-  uint32_t c, o, vmask, sh;  // key word, shift, value mask, 63 - shift
-  uint64_t mlo, mhi;         // field bits in word c, and in word c + 1 when it straddles
-};
-
-__device__ __forceinline__ FieldDesc field_desc(const uint32_t *M, uint32_t off, int p) {
-  const uint4 a = *reinterpret_cast<const uint4 *>(M + off + 8u * p);
-  const uint4 b = *reinterpret_cast<const uint4 *>(M + off + 8u * p + 4u);
-  return FieldDesc{a.x, a.y, a.z, a.w, static_cast<uint64_t>(b.x) | (static_cast<uint64_t>(b.y) << 32),
-                   static_cast<uint64_t>(b.z) | (static_cast<uint64_t>(b.w) << 32)};
-}
-
-template <int W>
-__device__ __forceinline__ uint32_t kget_d(const uint64_t (&k)[W], const FieldDesc &d) {
-  uint64_t lo = 0, hi = 0;
-#pragma unroll
-  for (int j = 0; j < W; j++) {
-    if (static_cast<uint32_t>(j) == d.c) lo = k[j];
-    if (static_cast<uint32_t>(j) == d.c + 1) hi = k[j];
-  }
-  uint64_t x = (lo & kPayload) >> d.o;
-  if (d.mhi) x |= (hi & kPayload) << d.sh;
-  return static_cast<uint32_t>(x) & d.vmask;
-}
-
-template <int W>
-__device__ __forceinline__ void kset_d(uint64_t (&k)[W], const FieldDesc &d, uint32_t v) {
-  const uint64_t x = static_cast<uint64_t>(v & d.vmask);
-  const uint64_t xlo = (x << d.o) & d.mlo;
-  const uint64_t xhi = d.mhi ? ((x >> d.sh) & d.mhi) : 0ull;
-#pragma unroll
-  for (int j = 0; j < W; j++) {
-    if (static_cast<uint32_t>(j) == d.c) k[j] = (k[j] & ~d.mlo) | xlo;
-    if (static_cast<uint32_t>(j) == d.c + 1) k[j] = (k[j] & ~d.mhi) | xhi;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // wave primitives
 __device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }

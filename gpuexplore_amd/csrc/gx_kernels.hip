@@ -183,7 +183,7 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
     acc_states += __popcll(ballot(has));
     if (has) {
       uint64_t pos = obase + off;
-      tps_emit<W>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
+      tps_emit<W, true>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
 #pragma unroll
         for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
         pos++;

@@ -271,6 +271,9 @@ int compile_model(const Gpf &g, Compiled *out) {
     rowrec.insert(rowrec.end(), {loc_start[r], loc_start[r + 1], syn_start[r], rowflags[r] | (labbits << kRowLabShift),
                                  rowlab[2 * r], rowlab[2 * r + 1], 0u, 0u});
   }
+  // row record nrows: a local state outside a process's rows reads this one
+  // (no entries, flagged bad) instead of branching around the read
+  rowrec.insert(rowrec.end(), {0u, 0u, 0u, static_cast<uint32_t>(kRowBad), 0u, 0u, 0u, 0u});
   for (size_t e = 0; e < syn_tstart.size(); e++) {
     synrec.push_back(syn_tstart[e]);
     synrec.push_back(syn_tcnt[e]);
@@ -283,12 +286,7 @@ int compile_model(const Gpf &g, Compiled *out) {
   for (uint32_t p = 0; p < N; p++) {
     const FieldLoc &f = out->floc[p];
     const uint64_t vmask = f.width >= 32 ? 0xFFFFFFFFull : ((1ull << f.width) - 1ull);
-    const uint64_t mlo = (vmask << f.offset) & kPayload;
-    const bool straddle = f.offset + f.width > 63;
-    const uint64_t mhi = straddle ? ((1ull << (f.offset + f.width - 63)) - 1ull) : 0ull;
-    fdesc.insert(fdesc.end(), {f.chunk, f.offset, static_cast<uint32_t>(vmask), 63u - f.offset,
-                               static_cast<uint32_t>(mlo), static_cast<uint32_t>(mlo >> 32),
-                               static_cast<uint32_t>(mhi), static_cast<uint32_t>(mhi >> 32)});
+    fdesc.insert(fdesc.end(), {f.chunk | (f.offset << 8), static_cast<uint32_t>(vmask), rowbase[p], rowcnt[p]});
   }
   while (B.size() % 4) B.push_back(0);
   h.off_fdesc = put(fdesc);

@@ -117,15 +117,46 @@ __device__ __forceinline__ uint32_t tps_en_of(const TpsInfo &inf, int i, const u
   return i == 0 ? inf.en[0] : i == 1 ? inf.en[1] : i == 2 ? inf.en[2] : inf.en[3];
 }
 
+// A process's record for the scan: where its field sits in the key, its
+// value mask, and its rows (ModelHdr::off_fdesc).
+struct ProcRec {
+  uint32_t c, o, vmask, rowbase, rowcnt;
+};
+
+__device__ __forceinline__ ProcRec proc_rec(const uint32_t *M, const ModelHdr &H, int p) {
+  const uint4 a = *reinterpret_cast<const uint4 *>(M + H.off_fdesc + 4u * static_cast<uint32_t>(p));
+  return ProcRec{a.x & 0xFFu, a.x >> 8, a.y, a.z, a.w};
+}
+
+// Local state of process p: the field's bits in word c from offset o, and
+// from bit 0 of word c+1 when the field straddles the two.  A field that
+// does not straddle has its width <= 63 - o, so whatever word c+1 adds
+// lands above the width and the value mask drops it: no straddle branch.
+template <int W>
+__device__ __forceinline__ uint32_t proc_state(const uint64_t (&key)[W], const ProcRec &f) {
+  uint64_t lo = 0, hi = 0;
+#pragma unroll
+  for (int j = 0; j < W; j++) {
+    if (static_cast<uint32_t>(j) == f.c) lo = key[j];
+    if (static_cast<uint32_t>(j) == f.c + 1) hi = key[j];
+  }
+  return static_cast<uint32_t>(((lo & kPayload) >> f.o) | ((hi & kPayload) << (63u - f.o))) & f.vmask;
+}
+
 // Pass 1 for the lane's state: rows, successor count, deadlock, safety.
+// The per-process loop has no branch on lane data: a local state outside
+// the process's rows reads the bad row record at index nrows, and lanes
+// without a state (has == false, key all zero) run it on local states 0
+// and have their results cleared at the end.
 template <int W, typename RowT>
 __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
                                             bool has, RowT *rows, int lane, bool safety) {
   const int N = static_cast<int>(H.nprocs);
-  const uint32_t fmask_last = field_desc(M, H.off_fdesc, N - 1).vmask;
+  const uint32_t fmask_last = proc_rec(M, H, N - 1).vmask;
   TpsInfo r{};
-  bool anyloc = false, anyen = false;
+  bool anyloc = false, anyen = false, bad = false;
   uint32_t multi = 0;  // processes whose row has a multi-target synchronising entry
+  uint32_t nloc = 0;   // local successors
   // A model with at most kTpsLabCache synchronising labels gets every
   // label's enabled set in this pass (men[k] for its k-th label), from the
   // label bits of the row records' flag words: the rows' 64-bit label masks
@@ -144,34 +175,31 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     }
   }
   for (int p = 0; p < N; p++) {
-    uint32_t row = kNoRow;
-    uint4 ra = make_uint4(0, 0, 0, 0);
-    if (has) {
-      const uint32_t sp = kget_d<W>(key, field_desc(M, H.off_fdesc, p));
-      if (sp < M[H.off_rowcnt + p]) {
-        row = M[H.off_rowbase + p] + sp;
-        ra = tps_row(M, H, row);
-        if (ra.w & kRowBad) row = kNoRow;
-      }
-      if (row == kNoRow) r.bad = true;
-    }
-    rows[p * 64 + lane] = static_cast<RowT>(row);
-    if (row != kNoRow) {
-      r.cnt += ra.y - ra.x;
-      if (ra.y > ra.x) r.lmask |= 1u << p;
-      anyloc |= (ra.w & kRowHasLocal) != 0;
-      if (ra.w & kRowMultiSync) multi |= 1u << p;
-      if (men_all) {  // the row's labels among the model's (at most four) from its flag word
+    const ProcRec f = proc_rec(M, H, p);
+    const uint32_t sp = proc_state<W>(key, f);
+    const uint32_t row = sp < f.rowcnt ? f.rowbase + sp : H.nrows;
+    const uint4 ra = tps_row(M, H, row);
+    const bool valid = (ra.w & kRowBad) == 0;
+    bad |= !valid;
+    rows[p * 64 + lane] = static_cast<RowT>(valid ? row : kNoRow);
+    const uint32_t w = valid ? ra.w : 0u;
+    const uint32_t nl = valid ? ra.y - ra.x : 0u;
+    nloc += nl;
+    r.lmask |= (nl != 0 ? 1u : 0u) << p;
+    anyloc |= (w & kRowHasLocal) != 0;
+    multi |= ((w & kRowMultiSync) != 0 ? 1u : 0u) << p;
+    if (men_all) {  // the row's labels among the model's (at most four) from its flag word
 #pragma unroll
-        for (int k = 0; k < kTpsLabCache; k++) men[k] |= ((ra.w >> (kRowLabShift + k)) & 1u) << p;
-      } else {
-        r.labs |= tps_rowlab(M, H, row);
-      }
-      if (safety && p == N - 1)
-        for (uint32_t t = ra.x; t < ra.y; t++)
-          if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) r.viol = true;
+      for (int k = 0; k < kTpsLabCache; k++) men[k] |= ((w >> (kRowLabShift + k)) & 1u) << p;
+    } else if (valid) {
+      r.labs |= tps_rowlab(M, H, row);
     }
+    if (safety && p == N - 1 && nl)
+      for (uint32_t t = ra.x; t < ra.y; t++)
+        if ((M[H.off_loc_tgt + t] & fmask_last) == 1u) r.viol = true;
   }
+  r.cnt = nloc;
+  r.bad = bad;
   if (men_all) {  // every synchronising label of the model is one of mlab[]
 #pragma unroll
     for (int k = 0; k < kTpsLabCache; k++)
@@ -239,7 +267,13 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
     if (i == 3) r.rmask[3] = rm;
   }
   r.dead = has && !anyloc && !anyen;
-  if (!has) r.cnt = 0;
+  if (!has) {
+    r.cnt = 0;
+    r.lmask = 0;
+    r.labs = 0;
+    r.viol = false;
+    r.bad = false;
+  }
   return r;
 }
 
@@ -277,11 +311,13 @@ __device__ __forceinline__ void tps_keyrec(const uint32_t *M, uint32_t off, uint
 // processes' fields cleared (a process's or a rule's key-shaped field mask)
 // OR the targets' key-shaped records, which the model holds already placed in
 // their fields.  Products are below 2^32 (they fit the successor buffer).
-template <int W, typename RowT, typename Emit>
+// EXACT: the model's key_words equal W (the single-model kernel), so the
+// record width is a constant and the record reads carry no width branches.
+template <int W, bool EXACT = false, typename RowT, typename Emit>
 __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, const uint64_t (&key)[W],
                                          const RowT *rows, int lane, const TpsInfo &inf, Emit &&emit) {
   const int N = static_cast<int>(H.nprocs);
-  const uint32_t kw = H.key_words;
+  const uint32_t kw = EXACT ? static_cast<uint32_t>(W) : H.key_words;
   // Local moves: each lane walks its own processes with local targets
   // (ascending, targets in order), so the wave takes as many steps as its
   // busiest lane has local successors, not the sum over processes of the
