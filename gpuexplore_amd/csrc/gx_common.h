@@ -53,18 +53,19 @@ struct ModelHdr {
   uint32_t off_rowlab;     // nrows x 2 words: mask of the synchronising labels of the row (nlab <= 64)
   uint32_t lab64;          // 1 when every label fits the 64-bit row masks
   // packed copies for the state-per-lane kernels (16-byte aligned offsets)
-  uint32_t off_rowrec;     // (nrows + 1) x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi, 0, 0
+  uint32_t off_rowrec;     // (nrows + 1) x 8 words: loc_start, loc_end, syn_start, flags, rowlab lo, hi,
+                           // and four 16-bit single-target delta indices (the model's first four
+                           // synchronising labels; kNoSyn16 when none or several)
   uint32_t off_synrec;     // per (row, label) entry x 2 words: tstart, tcnt
   uint32_t off_fdesc;      // per process x 4 words: key word | shift << 8, value mask, first
                            // row, rows (row records have one more, a bad one, at nrows)
-  uint32_t off_locx;       // per loc_tgt entry: the target placed in its field (key_words
-                           // 64-bit words as lo, hi pairs; zero outside the field)
-  uint32_t off_synx;       // per syn_tgt entry: the same for synchronising targets
-  uint32_t off_fclr;       // per process: its field bits (key-shaped)
-  uint32_t off_rclr;       // per rule: the field bits of its members (key-shaped)
+  uint32_t off_locx;       // per loc_tgt entry: key XOR this is the successor (the row's
+                           // state and the target, both placed in the process's field, XORed;
+                           // key_words 64-bit words as lo, hi pairs)
+  uint32_t off_synx;       // per syn_tgt entry: the same delta for synchronising targets
   uint32_t synlab_lo, synlab_hi;  // labels < 64 with a synchronising entry in some row
   uint32_t tps_begin;      // first word of the section the state-per-lane kernels read
-                           // (rowbase .. rclr, 4-word aligned, up to total_words)
+                           // (rowbase .. synx, 4-word aligned, up to total_words)
 };
 
 
@@ -74,6 +75,8 @@ enum RowFlags : uint32_t { kRowHasLocal = 1u, kRowBad = 2u, kRowMultiSync = 4u }
 // when the row has an entry for the model's k-th synchronising label (k < 4,
 // labels in ascending order).
 constexpr uint32_t kRowLabShift = 8;
+// no single-target synchronising delta for that label in the row record
+constexpr uint32_t kNoSyn16 = 0xFFFFu;
 
 // Exploration modes of the successor kernel.
 enum ExpandMode : int { kModePlain = 0, kModePor = 1, kModePorProviso = 2 };

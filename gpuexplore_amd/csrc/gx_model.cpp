@@ -268,8 +268,17 @@ int compile_model(const Gpf &g, Compiled *out) {
     uint32_t labbits = 0;
     for (int k = 0; k < 4; k++)
       if (first_labs[k] < 64u && ((rl >> first_labs[k]) & 1u)) labbits |= 1u << k;
+    // words 6-7: for the model's k-th synchronising label (k < 4), the
+    // index of the row's single target delta when its entry for the label
+    // has exactly one target (16 bits each, kNoSyn16 otherwise)
+    uint32_t one[4] = {kNoSyn16, kNoSyn16, kNoSyn16, kNoSyn16};
+    if (syn_tgt.size() < kNoSyn16) {
+      for (uint32_t e = syn_start[r]; e < syn_start[r + 1]; e++)
+        for (int k = 0; k < 4; k++)
+          if (syn_lab[e] == first_labs[k] && syn_tcnt[e] == 1) one[k] = syn_tstart[e];
+    }
     rowrec.insert(rowrec.end(), {loc_start[r], loc_start[r + 1], syn_start[r], rowflags[r] | (labbits << kRowLabShift),
-                                 rowlab[2 * r], rowlab[2 * r + 1], 0u, 0u});
+                                 rowlab[2 * r], rowlab[2 * r + 1], one[0] | (one[1] << 16), one[2] | (one[3] << 16)});
   }
   // row record nrows: a local state outside a process's rows reads this one
   // (no entries, flagged bad) instead of branching around the read
@@ -312,39 +321,35 @@ int compile_model(const Gpf &g, Compiled *out) {
     key_set(*out, k.data(), p, v);
     return k;
   };
-  std::vector<uint32_t> locx, synx, fclr, rclr;
-  // loc_tgt / syn_tgt entries belong to the process of their row
-  std::vector<uint32_t> row_proc(nrec, 0);
+  // A move of process p from local state s (its row's) to target v changes
+  // only p's field: the successor is key XOR (s placed ^ v placed).  One such
+  // delta per local and per synchronising target entry.
+  std::vector<uint32_t> locx, synx;
+  std::vector<uint32_t> row_proc(nrec, 0), row_state(nrec, 0);
   for (uint32_t p = 0; p < N; p++)
-    for (uint32_t r = rowbase[p]; r < rowbase[p] + rowcnt[p]; r++) row_proc[r] = p;
-  std::vector<uint32_t> syn_proc(syn_tstart.size(), 0);
+    for (uint32_t r = rowbase[p]; r < rowbase[p] + rowcnt[p]; r++) {
+      row_proc[r] = p;
+      row_state[r] = r - rowbase[p];
+    }
+  auto delta = [&](uint32_t r, uint32_t v) {
+    std::vector<uint64_t> a = placed(row_proc[r], row_state[r]), b = placed(row_proc[r], v);
+    for (uint32_t j = 0; j < W; j++) a[j] ^= b[j];
+    return a;
+  };
+  std::vector<uint32_t> syn_row(syn_tstart.size(), 0);
   for (uint32_t r = 0; r < nrec; r++) {
-    for (uint32_t t = loc_start[r]; t < loc_start[r + 1]; t++) put_key(locx, placed(row_proc[r], loc_tgt[t]));
-    for (uint32_t e = syn_start[r]; e < syn_start[r + 1]; e++) syn_proc[e] = row_proc[r];
+    for (uint32_t t = loc_start[r]; t < loc_start[r + 1]; t++) put_key(locx, delta(r, loc_tgt[t]));
+    for (uint32_t e = syn_start[r]; e < syn_start[r + 1]; e++) syn_row[e] = r;
   }
-  std::vector<uint32_t> tgt_proc(syn_tgt.size(), 0);
+  std::vector<uint32_t> tgt_row(syn_tgt.size(), 0);
   for (size_t e = 0; e < syn_tstart.size(); e++)
-    for (uint32_t t = syn_tstart[e]; t < syn_tstart[e] + syn_tcnt[e]; t++) tgt_proc[t] = syn_proc[e];
-  for (size_t t = 0; t < syn_tgt.size(); t++) put_key(synx, placed(tgt_proc[t], syn_tgt[t]));
-  for (uint32_t p = 0; p < N; p++) {
-    std::vector<uint64_t> k(W, 0);
-    field_bits(p, k);
-    put_key(fclr, k);
-  }
-  for (uint32_t r : rules) {
-    std::vector<uint64_t> k(W, 0);
-    for (uint32_t p = 0; p < N; p++)
-      if ((r >> p) & 1u) field_bits(p, k);
-    put_key(rclr, k);
-  }
+    for (uint32_t t = syn_tstart[e]; t < syn_tstart[e] + syn_tcnt[e]; t++) tgt_row[t] = syn_row[e];
+  for (size_t t = 0; t < syn_tgt.size(); t++) put_key(synx, delta(tgt_row[t], syn_tgt[t]));
   while (B.size() % 4) B.push_back(0);
   h.off_locx = put(locx);
   while (B.size() % 4) B.push_back(0);
   h.off_synx = put(synx);
-  while (B.size() % 4) B.push_back(0);
-  h.off_fclr = put(fclr);
-  while (B.size() % 4) B.push_back(0);
-  h.off_rclr = put(rclr);
+
   while (B.size() % 4) B.push_back(0);
   h.total_words = static_cast<uint32_t>(B.size());
   out->hdr = h;

@@ -19,10 +19,11 @@
 // walking every rule of a label would make each lane pay for the rules
 // enabled in any lane.  For the same reason each lane walks its own
 // processes with local targets (a bit mask from the scan) instead of the
-// wave stepping through every process.  A rule's product is 1 unless a member's row has a
-// synchronising entry with a target count other than 1 (kRowMultiSync), so
-// members are visited only then.  Successors are built from the model's
-// key-shaped records: (key & ~field mask) | target record.
+// wave stepping through every process.  A rule's product is 1 unless a
+// member's row has a synchronising entry with a target count other than 1
+// (kRowMultiSync), so members are visited only then.  Successors are the
+// key XOR the model's key-shaped target deltas; a rule whose members all
+// have one target finds each member's delta through the row record.
 #pragma once
 #include "gx_device.h"
 
@@ -44,8 +45,6 @@ __device__ __forceinline__ ModelHdr tps_rebased(const ModelHdr &h) {
   r.off_fdesc -= b;
   r.off_locx -= b;
   r.off_synx -= b;
-  r.off_fclr -= b;
-  r.off_rclr -= b;
   return r;
 }
 
@@ -96,6 +95,7 @@ struct TpsInfo {
   uint64_t labs;  // synchronising labels present in the state
   uint64_t cnt;   // successors
   uint32_t lmask; // processes whose row has local targets
+  uint32_t multi; // processes whose row has a synchronising entry with other than one target
   uint32_t en[kTpsLabCache];     // enabled sets of the first labels of labs (when cached)
   uint64_t rmask[kTpsLabCache];  // enabled rules of those labels, bit k = rule start + k
   bool cached;    // labs has at most kTpsLabCache labels
@@ -200,6 +200,7 @@ __device__ __forceinline__ TpsInfo tps_scan(const uint32_t *M, const ModelHdr &H
   }
   r.cnt = nloc;
   r.bad = bad;
+  r.multi = multi;
   if (men_all) {  // every synchronising label of the model is one of mlab[]
 #pragma unroll
     for (int k = 0; k < kTpsLabCache; k++)
@@ -307,10 +308,11 @@ __device__ __forceinline__ void tps_keyrec(const uint32_t *M, uint32_t off, uint
 }
 
 // Pass 2: calls emit(s) for every successor s of the lane's state, in the
-// order counted by tps_scan.  A successor is the key with the moving
-// processes' fields cleared (a process's or a rule's key-shaped field mask)
-// OR the targets' key-shaped records, which the model holds already placed in
-// their fields.  Products are below 2^32 (they fit the successor buffer).
+// order counted by tps_scan.  A move of process p from its current local
+// state to a target changes only p's field, so a successor is the key XOR
+// the moving processes' target deltas (the model holds one per target
+// entry: current and target state, both placed in p's field, XORed).
+// Products are below 2^32 (they fit the successor buffer).
 // EXACT: the model's key_words equal W (the single-model kernel), so the
 // record width is a constant and the record reads carry no width branches.
 template <int W, bool EXACT = false, typename RowT, typename Emit>
@@ -324,9 +326,6 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
   // busiest lane per process.
   {
     uint32_t lm = inf.lmask, t = 0, te = 0;
-    uint64_t base[W];
-#pragma unroll
-    for (int j = 0; j < W; j++) base[j] = 0;
     while (ballot(lm != 0 || t < te)) {
       if (t == te && lm) {
         const uint32_t p = static_cast<uint32_t>(__ffs(lm) - 1);
@@ -334,24 +333,25 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
         const uint4 ra = tps_row(M, H, row_at(rows, static_cast<int>(p) * 64 + lane));
         t = ra.x;
         te = ra.y;
-        tps_keyrec<W>(M, H.off_fclr, p, kw, base);
-#pragma unroll
-        for (int j = 0; j < W; j++) base[j] = key[j] & ~base[j];
       }
       if (t < te) {
         uint64_t s[W];
         tps_keyrec<W>(M, H.off_locx, t, kw, s);
 #pragma unroll
-        for (int j = 0; j < W; j++) s[j] |= base[j];
+        for (int j = 0; j < W; j++) s[j] ^= key[j];
         emit(s);
         t++;
       }
     }
   }
+  const uint64_t msyn = static_cast<uint64_t>(H.synlab_lo) | (static_cast<uint64_t>(H.synlab_hi) << 32);
+  const bool syn4 = __popcll(msyn) <= kTpsLabCache;  // row records hold single-target delta indices
   int i = 0;
   for (uint64_t L = inf.labs; L; L &= L - 1, i++) {
     const uint32_t l = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(L)) - 1);
     const uint32_t rs = M[H.off_rule_start + l], re = M[H.off_rule_start + l + 1];
+    // l is the model's k-th synchronising label: word 6 + k / 2, half k % 2
+    const uint32_t k = static_cast<uint32_t>(__popcll(msyn & ((1ull << l) - 1ull)));
     // the lane's enabled rules: from the scan's mask, else by walking the label's rules
     const uint32_t en = inf.rcached ? 0u : tps_en_of(inf, i, M, H, rows, lane, N, l);
     uint64_t rm = inf.rcached ? tps_rmask_of(inf, i) : 0ull;
@@ -362,14 +362,27 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
       }
       const uint32_t rule = M[H.off_rules + j];
       if (!inf.rcached && (rule & ~en)) continue;
-      uint64_t base[W];
-      tps_keyrec<W>(M, H.off_rclr, j, kw, base);
-#pragma unroll
-      for (int jj = 0; jj < W; jj++) base[jj] = key[jj] & ~base[jj];
-      // combination 0 (every member at its first target) with the product
       uint64_t s[W];
 #pragma unroll
-      for (int jj = 0; jj < W; jj++) s[jj] = base[jj];
+      for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
+      if (syn4 && (rule & inf.multi) == 0) {
+        // every member has exactly one target: one successor, each member's
+        // delta found from its row record
+        for (uint32_t rr = rule; rr; rr &= rr - 1) {
+          const int q = __ffs(rr) - 1;
+          const uint32_t row = row_at(rows, q * 64 + lane);
+          const uint32_t w = M[H.off_rowrec + 8u * row + 6u + (k >> 1)];
+          uint32_t d = (w >> (16u * (k & 1u))) & 0xFFFFu;
+          if (d == kNoSyn16) d = tps_syn(M, H, row, l).x;  // a model with 2^16 or more such targets
+          uint64_t x[W];
+          tps_keyrec<W>(M, H.off_synx, d, kw, x);
+#pragma unroll
+          for (int jj = 0; jj < W; jj++) s[jj] ^= x[jj];
+        }
+        emit(s);
+        continue;
+      }
+      // combination 0 (every member at its first target) with the product
       uint32_t prod = 1;
       for (uint32_t rr = rule; rr; rr &= rr - 1) {
         const int q = __ffs(rr) - 1;
@@ -379,15 +392,15 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
           uint64_t x[W];
           tps_keyrec<W>(M, H.off_synx, e.x, kw, x);
 #pragma unroll
-          for (int jj = 0; jj < W; jj++) s[jj] |= x[jj];
+          for (int jj = 0; jj < W; jj++) s[jj] ^= x[jj];
         }
       }
       if (prod == 0) continue;  // a member without targets: the rule yields nothing
       emit(s);
-      for (uint32_t k = 1; k < prod; k++) {  // lowest process varies fastest
+      for (uint32_t c = 1; c < prod; c++) {  // lowest process varies fastest
 #pragma unroll
-        for (int jj = 0; jj < W; jj++) s[jj] = base[jj];
-        uint32_t rem = k;
+        for (int jj = 0; jj < W; jj++) s[jj] = key[jj];
+        uint32_t rem = c;
         for (uint32_t rr = rule; rr; rr &= rr - 1) {
           const int q = __ffs(rr) - 1;
           const uint2 e = tps_syn(M, H, row_at(rows, q * 64 + lane), l);
@@ -396,7 +409,7 @@ __device__ __forceinline__ void tps_emit(const uint32_t *M, const ModelHdr &H, c
           uint64_t x[W];
           tps_keyrec<W>(M, H.off_synx, e.x + d, kw, x);
 #pragma unroll
-          for (int jj = 0; jj < W; jj++) s[jj] |= x[jj];
+          for (int jj = 0; jj < W; jj++) s[jj] ^= x[jj];
         }
         emit(s);
       }
