@@ -230,6 +230,105 @@ def summarize(d: str, name: str, kernel: str, out_dir: str, roof_path: str = "ro
     return data
 
 
+# SQ instruction and cycle counters of the level kernels, in passes of at
+# most eight SQ counters each
+SQ_PASSES = [
+    ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VALU_INT64", "SQ_INSTS_VALU_INT32",
+     "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES"],
+    ["SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "SQ_INSTS_BRANCH",
+     "SQ_INSTS_VMEM_WR", "SQ_INSTS_VMEM_RD", "SQ_CYCLES"],
+    ["SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE", "SQ_WAIT_ANY", "SQ_INSTS_SMEM"],
+]
+
+
+def collect_sq(out: str, cmd: List[str], limit: int) -> int:
+    env = dict(os.environ, TMPDIR="/tmp")
+    os.makedirs(out, exist_ok=True)
+    for i, counters in enumerate(SQ_PASSES):
+        name = f"sq{i + 1}"
+        print(f"profile pass {name}", flush=True)
+        full = ["timeout", "-s", "KILL", str(limit), "rocprofv3", "--pmc", *counters, "-d", os.path.join(out, name),
+                "-o", name, "--output-format", "csv", "--", *cmd]
+        with open(os.path.join(out, f"{name}.stdout"), "w") as so, open(os.path.join(out, f"{name}.stderr"), "w") as se:
+            rc = subprocess.run(full, cwd="/tmp", env=env, stdout=so, stderr=se).returncode
+        done = any(ln.startswith("{") for ln in open(os.path.join(out, f"{name}.stdout")).read().splitlines())
+        if rc != 0 and not (rc in (139, -11) and done):
+            print(f"pass {name} failed with {rc}", file=sys.stderr)
+            return rc or 1
+    return 0
+
+
+def summarize_sq(d: str, name: str, out_dir: str, kernels=("k_expand_tps", "k_insert"), cmd: str = BENCH_CMD) -> Dict:
+    """SQ counters of the two level kernels summed over their launches, with
+    per-unit figures: per 64-state batch (successor kernel) and per
+    16-candidate iteration (FIND-OR-PUT), from the bench line's counts."""
+    bench = None
+    for ln in open(os.path.join(d, "sq1.stdout")).read().splitlines():
+        if ln.startswith("{"):
+            bench = json.loads(ln)
+    tot = defaultdict(lambda: defaultdict(float))
+    for i in range(len(SQ_PASSES)):
+        per, _ = _counters(_one(os.path.join(d, f"sq{i + 1}", "**", "*counter_collection.csv")))
+        for k, v in per.items():
+            for c, x in v.items():
+                tot[k][c] += x
+    names = {kk: next((k for k in tot if kk in k and "<" in k), None) for kk in kernels}
+    states = bench["config"]["states"]
+    cand = bench["hash_phase"]["candidates"]
+    batches = states / 64.0
+    iters = cand / 16.0
+    cus, clk = 256, 2.4e9
+    md = [f"# SQ counters: {' and '.join(kernels)} on {bench['config']['workload']}", "",
+          f"{len(SQ_PASSES)} `rocprofv3 --pmc` passes of one exploration (counters summed over each kernel's "
+          "launches), from the repository root:", "", "```",
+          f"python3 -m gpuexplore_amd.profsum collect-sq --out <dir> -- {cmd}",
+          f"python3 -m gpuexplore_amd.profsum summarize-sq --dir <dir> --name {name}", "```", "",
+          "| counter | " + " | ".join(f"`{names[k]}`" for k in kernels) + " |",
+          "|---|" + "---|" * len(kernels)]
+    allc = sorted({c for k in kernels if names[k] for c in tot[names[k]]})
+    for c in allc:
+        md.append(f"| {c} | " + " | ".join(f"{tot[names[k]].get(c, 0):.4g}" if names[k] else "-" for k in kernels) + " |")
+    derived = {}
+    e = tot[names[kernels[0]]] if names[kernels[0]] else {}
+    f = tot[names[kernels[1]]] if names[kernels[1]] else {}
+    if e:
+        derived["expand_per_batch"] = {c: e.get(c, 0) / batches for c in
+                                       ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_BRANCH")}
+        derived["expand_valu_ms"] = e.get("SQ_INSTS_VALU", 0) / (cus * clk) * 1e3
+        derived["expand_lanes_active"] = (e.get("SQ_THREAD_CYCLES_VALU", 0) / e["SQ_ACTIVE_INST_VALU"]
+                                          if e.get("SQ_ACTIVE_INST_VALU") else None)
+        derived["expand_lds_bank_conflict_share"] = (e.get("SQ_LDS_BANK_CONFLICT", 0) / e["SQ_LDS_IDX_ACTIVE"]
+                                                     if e.get("SQ_LDS_IDX_ACTIVE") else None)
+    if f:
+        derived["insert_per_iteration"] = {c: f.get(c, 0) / iters for c in
+                                           ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS")}
+        derived["insert_valu_ms"] = f.get("SQ_INSTS_VALU", 0) / (cus * clk) * 1e3
+    md += ["", f"Derived ({states:,} states, {cand:,} successors; a wave instruction issues at one per CU cycle, "
+               f"{cus} CUs x {clk / 1e9:.1f} GHz):", ""]
+    if e:
+        pb = derived["expand_per_batch"]
+        md.append(f"- {kernels[0]} per 64-state batch: {pb['SQ_INSTS_VALU']:,.0f} VALU, {pb['SQ_INSTS_SALU']:,.0f} SALU, "
+                  f"{pb['SQ_INSTS_LDS']:,.0f} LDS, {pb['SQ_INSTS_BRANCH']:,.0f} branch instructions; "
+                  f"{derived['expand_valu_ms']:.1f} ms of VALU issue; "
+                  + (f"{derived['expand_lanes_active']:.1f} of 64 lanes active per VALU instruction; "
+                     if derived["expand_lanes_active"] else "")
+                  + (f"LDS bank conflict cycles {100 * derived['expand_lds_bank_conflict_share']:.0f} % of LDS "
+                     "cycles (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE)." if derived["expand_lds_bank_conflict_share"]
+                     is not None else ""))
+    if f:
+        pi = derived["insert_per_iteration"]
+        md.append(f"- {kernels[1]} per 16-candidate iteration (4 lanes per candidate): {pi['SQ_INSTS_VALU']:.0f} VALU, "
+                  f"{pi['SQ_INSTS_SALU']:.0f} SALU, {pi['SQ_INSTS_LDS']:.1f} LDS instructions; "
+                  f"{derived['insert_valu_ms']:.1f} ms of VALU issue.")
+    md.append(f"- Kernel times without counters (the bench line): {kernels[0]} "
+              f"{1e3 * bench['hash_phase']['t_expand_s_per_step']:.1f} ms, {kernels[1]} "
+              f"{1e3 * bench['hash_phase']['t_hash_s_per_step']:.1f} ms per exploration.")
+    os.makedirs(out_dir, exist_ok=True)
+    with open(os.path.join(out_dir, f"{name}.md"), "w") as fh:
+        fh.write("\n".join(md) + "\n")
+    return {"counters": {k: dict(tot[names[k]]) for k in kernels if names[k]}, "derived": derived}
+
+
 def gaps(d: str, prefix: str = "k_batch", split_ns: int = 2_000_000) -> List[Dict]:
     """Idle time between the batch's kernels in the kernel-trace run: the
     launches whose names start with `prefix`, split into explorations where
@@ -274,10 +373,24 @@ def main(argv=None) -> int:
     s.add_argument("--profiles", default=os.path.join(ROOT, "profiles"))
     s.add_argument("--roof", default="roofline", help="path of the roofline object in the bench line")
     s.add_argument("--cmd", default=BENCH_CMD, help="the profiled command, as recorded in the summary")
+    cs = sub.add_parser("collect-sq")
+    cs.add_argument("--out", required=True)
+    cs.add_argument("--limit", type=int, default=300)
+    cs.add_argument("rest", nargs=argparse.REMAINDER)
+    ss = sub.add_parser("summarize-sq")
+    ss.add_argument("--dir", required=True)
+    ss.add_argument("--name", required=True)
+    ss.add_argument("--profiles", default=os.path.join(ROOT, "profiles"))
     g = sub.add_parser("gaps")
     g.add_argument("--dir", required=True)
     g.add_argument("--prefix", default="k_batch")
     a = ap.parse_args(argv)
+    if a.cmd == "collect-sq":
+        rest = a.rest[1:] if a.rest and a.rest[0] == "--" else a.rest
+        return collect_sq(os.path.abspath(a.out), rest, a.limit)
+    if a.cmd == "summarize-sq":
+        print(json.dumps(summarize_sq(os.path.abspath(a.dir), a.name, a.profiles)["derived"], indent=1))
+        return 0
     if a.cmd == "gaps":
         for r in gaps(os.path.abspath(a.dir), a.prefix):
             print(json.dumps(r))
