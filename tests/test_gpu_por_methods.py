@@ -126,3 +126,23 @@ def test_bad_property_rejected(gpu, model_dir):
     with pytest.raises(gpu.GxError) as e:
         srv.submit(m, 9, est_states=10)
     assert e.value.status == -7
+
+
+# config-1 rings on which the cycle proviso changes the reduced count 5-30x
+# (cample, no property: seed 4 4,404 -> 57,989 states, seed 5 11,639 ->
+# 127,048, seed 7 7,848 -> 95,576, seed 8 903 -> 12,925)
+@pytest.mark.parametrize("seed", [4, 5, 7, 8])
+def test_rings_where_the_proviso_matters(gpu, oracle_mod, model_dir, seed):
+    p = os.path.join(model_dir, f"ring{seed}")
+    gen.config1(seed).write(p)
+    for method in METHODS:
+        for prop in (0, 1):
+            counts = {}
+            for prov in (False, True):
+                say("ring", seed, method, prop, prov)
+                a = oracle_mod.explore(p, prop, True, prov, method=METHODS[method])
+                b = gpu_run(gpu, p, prop, True, prov, por_method=METHODS[method])
+                assert same(a, b), (seed, method, prop, prov, a, b)
+                counts[prov] = b["states"]
+            if prop == 0:
+                assert counts[True] > counts[False], (seed, method, counts)
