@@ -320,11 +320,14 @@ def main() -> int:
     if os.path.exists(prof) and args.q_log2 == 34 and args.seed == 1 and not args.bucket_bytes \
             and not args.expand_kernel:
         pj = json.load(open(prof))
-        for k, v in pj["kernels"].items():
-            if "k_insert" in k:
-                traffic = v["bytes_per_launch"]
-                physical = v.get("dram_bytes_per_launch")
-                atomics = v.get("atomics_per_launch")
+        # the level passes' instantiation (the initial state's one-wave pass
+        # may be another one)
+        ins = [v for k, v in pj["kernels"].items() if "k_insert" in k]
+        if ins:
+            v = max(ins, key=lambda v: v["level_launches"])
+            traffic = v["bytes_per_launch"]
+            physical = v.get("dram_bytes_per_launch")
+            atomics = v.get("atomics_per_launch")
 
     del ex  # the config 2 buffers (a 2^34-int visited set) are not needed any more
     batch = None if args.no_batch else batched_vs_plain(gx, gen, tmp, args.batch, not args.no_autotune)

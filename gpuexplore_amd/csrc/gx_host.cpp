@@ -395,6 +395,15 @@ extern "C" int gx_explorer_run(gx_explorer *e, gx_result *res) {
   // initial state: FIND-OR-PUT of s0 into frontier 0 with stamp 0
   GX_TRY(launch_insert(ia, W, WT, BW, e->stamp, 1, 64, st));
   GX_TRY(launch_advance(ia.ctrl, st));
+  // level passes hand successors over as 12-byte candidates where keys have
+  // two words of at most kPackedMaxBits state bits (the initial state above
+  // went in the 16-byte form)
+  {
+    const int pk = (e->mode == kModePlain && e->per_state && W == 2 && WT == 2 && !e->stamp && !opt->dedup &&
+                    H.state_bits <= kPackedMaxBits) ? 1 : 0;
+    ea.packed = pk;
+    ia.packed = pk;
+  }
   GX_TRY(hipStreamSynchronize(st));
   const auto t_init1 = std::chrono::steady_clock::now();
   res->t_init_s = std::chrono::duration<double>(t_init1 - t_init0).count();

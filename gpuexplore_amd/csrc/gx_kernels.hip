@@ -120,7 +120,7 @@ __global__ __launch_bounds__(512) void k_expand(ExpandArgs a) {
 // state) a lane expanding a whole state is cheaper than a group of N lanes:
 // 64 states per wave instead of floor(64/N), and no cross-lane traffic
 // except the wave's output reservation (gx_tps.h has the per-state code).
-template <int W, bool LDS, typename RowT>
+template <int W, bool LDS, typename RowT, bool PACKED = false>
 __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   extern __shared__ uint32_t smem[];
   // with LDS, only the state-per-lane section of the model is staged
@@ -151,6 +151,7 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   uint64_t acc_trans = 0, acc_states = 0;
   bool viol = false, bad = false;
   CandChunk ch(a.cand, a.cand_cap, &C->cand_count);
+  ch.packed = PACKED;
 
   for (uint64_t b0 = gw * 64; b0 < nwork; b0 += nw * 64) {
     const uint64_t si = b0 + lane;
@@ -184,8 +185,12 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
     if (has) {
       uint64_t pos = obase + off;
       tps_emit<W, true>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
+        if constexpr (PACKED) {
+          pack12(sv, reinterpret_cast<uint32_t *>(a.cand) + 3 * pos);
+        } else {
 #pragma unroll
-        for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+          for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
+        }
         pos++;
       });
     }
@@ -217,6 +222,8 @@ template <int W, int WT, int BW, bool STAMP, int OPT>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kInsertMinWaves<WT, STAMP>)))
 void k_insert(InsertArgs a) {
   constexpr bool DEDUP = (OPT & kInsDedup) != 0;
+  constexpr bool PACKED = (OPT & kInsPacked) != 0;
+  static_assert(!PACKED || (W == 2 && WT == 2 && !STAMP), "packed candidates hold 2-word keys");
   constexpr uint64_t fl = (OPT & kInsSetFlag) ? kFlag : 0ull;
   extern __shared__ uint64_t stage_all[];
   Ctrl *C = a.ctrl;
@@ -299,9 +306,18 @@ void k_insert(InsertArgs a) {
         c0 = r0;
       }
       uint64_t key[W];
+      if constexpr (PACKED) {
+        const uint32_t *c32 = reinterpret_cast<const uint32_t *>(a.cand) + 3 * ci;
+        const uint32_t u2 = ci < n ? __builtin_nontemporal_load(c32 + 2) : 0u;
+        const uint32_t u0 = ci < n ? __builtin_nontemporal_load(c32) : 0u;
+        const uint32_t u1 = ci < n ? __builtin_nontemporal_load(c32 + 1) : 0u;
+        unpack12(u0, u1, u2, key);
+        if (!(u2 >> 31)) key[0] = 0;
+      } else {
 #pragma unroll
-      for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
-        key[j] = ci < n ? (__builtin_nontemporal_load(a.cand + ci * W + j) | fl) : 0;
+        for (int j = 0; j < W; j++)  // read once: keep the table's lines in the caches
+          key[j] = ci < n ? (__builtin_nontemporal_load(a.cand + ci * W + j) | fl) : 0;
+      }
       // a zero first word: past the end or a released reservation
       const uint64_t h = key_hash<W>(key, a.table.seed);
       visit(ci, key, h, bucket_of(h, 0, a.table.nbuckets), key[0] != 0);
@@ -340,8 +356,17 @@ void k_insert(InsertArgs a) {
         c0 = r0;
       }
       uint64_t key_l[W];
+      if constexpr (PACKED) {
+        const uint32_t *c32 = reinterpret_cast<const uint32_t *>(a.cand) + 3 * cl;
+        const uint32_t u0 = cl < n ? __builtin_nontemporal_load(c32) : 0u;
+        const uint32_t u1 = cl < n ? __builtin_nontemporal_load(c32 + 1) : 0u;
+        const uint32_t u2 = cl < n ? __builtin_nontemporal_load(c32 + 2) : 0u;
+        unpack12(u0, u1, u2, key_l);
+        if (!(u2 >> 31)) key_l[0] = 0;
+      } else {
 #pragma unroll
-      for (int j = 0; j < W; j++) key_l[j] = cl < n ? (__builtin_nontemporal_load(a.cand + cl * W + j) | fl) : 0;
+        for (int j = 0; j < W; j++) key_l[j] = cl < n ? (__builtin_nontemporal_load(a.cand + cl * W + j) | fl) : 0;
+      }
       uint64_t act_m = ballot(key_l[0] != 0);
       const uint64_t h_l = key_hash<W>(key_l, a.table.seed);
       const uint64_t b_l = bucket_of(h_l, 0, a.table.nbuckets);
@@ -512,6 +537,20 @@ static hipError_t expand_w(const ExpandArgs &a, int mode, bool lds, uint32_t blo
     const size_t sec_b = lds ? ((a.hdr.total_words - a.hdr.tps_begin + 3u) & ~3u) * 4u : 0u;
     const bool r16 = a.hdr.nrows < 0xFFFFu;
     const size_t shm = sec_b + expand_tps_scratch_bytes(threads, a.hdr.nprocs, r16);
+    if constexpr (W == 2 && WT == 2) {
+      if (a.packed) {
+        if (lds && r16)
+          hipLaunchKernelGGL((k_expand_tps<W, true, uint16_t, true>), dim3(blocks), dim3(threads), shm, s, a);
+        else if (lds)
+          hipLaunchKernelGGL((k_expand_tps<W, true, uint32_t, true>), dim3(blocks), dim3(threads), shm, s, a);
+        else if (r16)
+          hipLaunchKernelGGL((k_expand_tps<W, false, uint16_t, true>), dim3(blocks), dim3(threads), shm, s, a);
+        else
+          hipLaunchKernelGGL((k_expand_tps<W, false, uint32_t, true>), dim3(blocks), dim3(threads), shm, s, a);
+        return hipGetLastError();
+      }
+    }
+    if (a.packed) return hipErrorInvalidValue;
     if (lds && r16)
       hipLaunchKernelGGL((k_expand_tps<W, true, uint16_t>), dim3(blocks), dim3(threads), shm, s, a);
     else if (lds)
@@ -576,6 +615,12 @@ template <int W, int WT, bool STAMP>
 static hipError_t insert_bw(const InsertArgs &a, int bw, uint32_t blocks, uint32_t threads, hipStream_t s) {
   if (a.set_flag) {
     if constexpr (W == WT && !STAMP) return insert_bw_opt<W, WT, STAMP, kInsSetFlag>(a, bw, blocks, threads, s);
+    return hipErrorInvalidValue;
+  }
+  if (a.packed) {
+    if constexpr (W == 2 && WT == 2 && !STAMP) {
+      if (!a.dedup && a.append && !a.is_new) return insert_bw_opt<W, WT, STAMP, kInsPacked>(a, bw, blocks, threads, s);
+    }
     return hipErrorInvalidValue;
   }
   if (a.dedup) return insert_bw_opt<W, WT, STAMP, kInsDedup>(a, bw, blocks, threads, s);

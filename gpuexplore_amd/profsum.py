@@ -215,7 +215,9 @@ def summarize(d: str, name: str, kernel: str, out_dir: str, roof_path: str = "ro
         md.append(f"| `{k}` | {v['level_launches']} | {v['read_bytes'] / 1e9:.1f} | {v['write_bytes'] / 1e9:.1f} | "
                   f"{v['bytes_per_launch'] / 1e9:.2f} | {v['req_64b'] / 1e9:.2f} G / {v['req_128b'] / 1e9:.2f} G | "
                   f"{v['atomics_per_launch'] / 1e6:.1f} M |")
-    ki = next((v for k, v in kern.items() if kernel in k), None)
+    # the level passes' instantiation: the one with the most level launches
+    # (the single-model initial state goes through another, one-wave one)
+    ki = max((v for k, v in kern.items() if kernel in k), key=lambda v: v["level_launches"], default=None)
     if ki and avg_us:
         phys = ki["gbps"]
         md += ["", f"Algorithmic bytes per {kernel} launch (128 B x (lookups + inserts)): "
@@ -272,7 +274,9 @@ def summarize_sq(d: str, name: str, out_dir: str, kernels=("k_expand_tps", "k_in
         for k, v in per.items():
             for c, x in v.items():
                 tot[k][c] += x
-    names = {kk: next((k for k in tot if kk in k and "<" in k), None) for kk in kernels}
+    # per kernel, the instantiation with the most waves (the level passes)
+    names = {kk: max((k for k in tot if kk in k and "<" in k), key=lambda k: tot[k].get("SQ_WAVES", 0), default=None)
+             for kk in kernels}
     states = bench["config"]["states"]
     cand = bench["hash_phase"]["candidates"]
     batches = states / 64.0

@@ -293,3 +293,25 @@ def test_dedup_filter_invariance(gpu, oracle_mod, model_dir):
         [runs[1][k] for k in ("states", "transitions", "levels")]
     assert runs[0]["lookups"] == runs[0]["transitions"] + 1  # + the initial state's
     assert runs[1]["lookups"] < 0.97 * runs[0]["lookups"], (runs[0]["lookups"], runs[1]["lookups"])
+
+
+# Two-word keys around the 12-byte candidate limit (95 state bits): plain
+# explorations of such models hand successors over packed, wider ones in
+# 16-byte form.  Fields straddle the 32-bit and the 63-bit word boundaries.
+@pytest.mark.parametrize("widths", [[20, 32, 32, 11], [31, 32, 32], [32, 32, 32], [31, 32, 32, 31]],
+                         ids=["95_straddle", "95", "96", "126"])
+def test_two_word_keys_around_the_packed_limit(gpu, oracle_mod, model_dir, widths):
+    net = gen.throttled(len(widths), 6, 2, 2, (0.5, 0.4, 0.1), 2, 7)
+    net.field_widths = widths
+    p = os.path.join(model_dir, "w2")
+    net.write(p)
+    info = gpu.Model.load(p).info()
+    assert info["key_words"] == 2 and info["state_bits"] == sum(widths)
+    for prop in (0, 1, 2):
+        for por, prov in ((False, False), (True, False), (True, True)):
+            say("two-word keys", widths, prop, por, prov)
+            a = oracle_mod.explore(p, prop, por, prov)
+            b = gpu_run(gpu, p, prop, por, prov)
+            assert same(a, b), (widths, prop, por, prov, a, b)
+            if not por:  # the same counts through the duplicate filter (16-byte candidates)
+                assert same(a, gpu_run(gpu, p, prop, dedup=1)), (widths, prop)
