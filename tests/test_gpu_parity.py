@@ -315,3 +315,20 @@ def test_two_word_keys_around_the_packed_limit(gpu, oracle_mod, model_dir, width
             assert same(a, b), (widths, prop, por, prov, a, b)
             if not por:  # the same counts through the duplicate filter (16-byte candidates)
                 assert same(a, gpu_run(gpu, p, prop, dedup=1)), (widths, prop)
+
+
+@pytest.mark.parametrize("cap", [2048, 4096])
+def test_packed_candidates_retry_path(gpu, oracle_mod, model_dir, cap):
+    """12-byte candidates when a level's successors do not fit the candidate
+    buffer: the 6-process ring of 64,512 states with its fields widened to 95
+    state bits, levels of thousands of states, a buffer of a few batches"""
+    net = gen.config1(2)
+    net.field_widths = [15, 16, 16, 16, 16, 16]
+    p = os.path.join(model_dir, "ring95")
+    net.write(p)
+    assert gpu.Model.load(p).info()["state_bits"] == 95
+    for prop in (0, 1):
+        say("packed retry", cap, prop)
+        a = oracle_mod.explore(p, prop)
+        b = gpu_run(gpu, p, prop, cand_capacity=cap)
+        assert same(a, b), (cap, prop, a, b)
