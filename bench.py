@@ -350,8 +350,10 @@ def main() -> int:
         threads = max(1, min(CPU_CORES, len(os.sched_getaffinity(0))))
         om = oracle.OracleModel(path, False)
         full_cpu = args.cpu_sample_states == 0
+        print(f"CPU baseline: oracle BFS on {threads} threads", file=sys.stderr, flush=True)
         mt = om.explore_mt(threads, oracle.PROP_NONE, max_states=args.cpu_sample_states,
-                           expected_states=states if full_cpu else args.cpu_sample_states)
+                           expected_states=states if full_cpu else args.cpu_sample_states, progress=True)
+        print("CPU baseline: one thread, first levels", file=sys.stderr, flush=True)
         st = om.explore(oracle.PROP_NONE, max_states=args.cpu_single_states)
         if full_cpu and (mt["states"], mt["transitions"]) != (states, runs[0]["transitions"]):
             print("CPU baseline counts differ from the GPU's", file=sys.stderr)

@@ -969,6 +969,10 @@ int orc_explore_mt(const orc_model *m, const orc_options *o, int nthreads, orc_r
     cur = nxt;
     level++;
     res->levels = level;
+    if (o->progress) {
+      fprintf(stderr, "oracle: level %u done, %llu states\n", level, (unsigned long long)states);
+      fflush(stderr);
+    }
     if (violation) break; /* level-granular early stop */
     if (o->max_states && states >= o->max_states) {
       res->truncated = 1;

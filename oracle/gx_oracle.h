@@ -90,6 +90,7 @@ typedef struct orc_options {
   uint64_t hash_seed;
   int por_method;     /* ORC_POR_*, used when por is set */
   uint64_t expected_states; /* presizes the visited set (0: grow from a small table) */
+  int progress;       /* orc_explore_mt: a line on stderr after each level */
 } orc_options;
 
 typedef struct orc_result {

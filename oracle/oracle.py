@@ -28,6 +28,7 @@ class _Opts(ctypes.Structure):
         ("hash_seed", ctypes.c_uint64),
         ("por_method", ctypes.c_int),
         ("expected_states", ctypes.c_uint64),
+        ("progress", ctypes.c_int),
     ]
 
 
@@ -189,10 +190,11 @@ class OracleModel:
                     truncated=bool(r.truncated), seconds=r.seconds)
 
     def explore_mt(self, threads: int, prop: int = PROP_NONE, max_states: int = 0, seed: int = 0,
-                   expected_states: int = 0) -> dict:
+                   expected_states: int = 0, progress: bool = False) -> dict:
         """orc_explore_mt: the plain method on `threads` threads (the visited
-        set presized for `expected_states` when given)."""
-        o = _Opts(prop, 0, 0, max_states, seed, POR_CAMPLE, expected_states)
+        set presized for `expected_states` when given; with `progress`, a
+        line on stderr after each level)."""
+        o = _Opts(prop, 0, 0, max_states, seed, POR_CAMPLE, expected_states, int(progress))
         r = _Res()
         rc = lib().orc_explore_mt(self.ptr, ctypes.byref(o), threads, ctypes.byref(r))
         if rc != 0:
