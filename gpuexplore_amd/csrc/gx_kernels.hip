@@ -183,7 +183,14 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
     acc_trans += total;
     acc_states += __popcll(ballot(has));
     if (has) {
-      uint64_t pos = obase + off;
+      // Even lanes fill their range from its end. A line that a lane shares
+      // with its right-hand neighbour is then written by both in their first
+      // steps, and one shared with its left-hand neighbour in their last
+      // steps, so no line stays partly written, and held in the L2, for a
+      // whole batch (successor generation on config 2: 31.3 -> 27.7 ms).
+      const bool rev = (lane & 1) == 0;
+      uint64_t pos = rev ? obase + off + inf.cnt - 1 : obase + off;
+      const uint64_t step = rev ? ~0ull : 1ull;  // -1 or +1, modulo 2^64
       tps_emit<W, true>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
         if constexpr (PACKED) {
           pack12(sv, reinterpret_cast<uint32_t *>(a.cand) + 3 * pos);
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
 #pragma unroll
           for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
         }
-        pos++;
+        pos += step;
       });
     }
   }
