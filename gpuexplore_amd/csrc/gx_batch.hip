@@ -148,11 +148,15 @@ __device__ __forceinline__ void batch_chunk(const BatchArgs &a, BatchCtrl *C, Bl
   }
   acc.add(a.inst, i, total, static_cast<uint64_t>(__popcll(ballot(has))));
   if (has) {
-    uint64_t pos = obase + off;
+    // even lanes fill their range from its end, as in k_expand_tps: lines
+    // shared by neighbouring lanes complete within a few steps
+    const bool rev = (lane & 1) == 0;
+    uint64_t pos = rev ? obase + off + inf.cnt - 1 : obase + off;
+    const uint64_t step = rev ? ~0ull : 1ull;  // -1 or +1, modulo 2^64
     tps_emit<W>(M, H, key, rows, lane, inf, [&](const uint64_t (&sv)[W]) {
 #pragma unroll
       for (int j = 0; j < W; j++) a.cand[pos * W + j] = sv[j];
-      pos++;
+      pos += step;
     });
   }
 }
