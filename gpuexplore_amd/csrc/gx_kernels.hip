@@ -211,6 +211,15 @@ __global__ __launch_bounds__(512) void k_expand_tps(ExpandArgs a) {
   if (ballot(bad) && lane == 0) C->bad_state = 1;
 }
 
+// Raw buffer loads (the llvm.amdgcn.raw.buffer.load intrinsic): a buffer
+// resource is base address (48 bits), stride 0, size in bytes and the
+// gfx9 format word 0x00020000; reads past the size return zeros. aux bit 1
+// is the non-temporal hint.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x3 __attribute__((ext_vector_type(3)));
+__device__ i32x3 raw_buffer_load_v3(i32x4 rsrc, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.v3i32");
+
 // ---------------------------------------------------------------------------
 // FIND-OR-PUT over the candidate buffer.  New keys are staged per wave in
 // LDS and appended to the next frontier in blocks of up to kStage keys, one
@@ -364,10 +373,20 @@ void k_insert(InsertArgs a) {
       }
       uint64_t key_l[W];
       if constexpr (PACKED) {
-        const uint32_t *c32 = reinterpret_cast<const uint32_t *>(a.cand) + 3 * cl;
-        const uint32_t u0 = cl < n ? __builtin_nontemporal_load(c32) : 0u;
-        const uint32_t u1 = cl < n ? __builtin_nontemporal_load(c32 + 1) : 0u;
-        const uint32_t u2 = cl < n ? __builtin_nontemporal_load(c32 + 2) : 0u;
+        // the step's 64 candidates through one buffer resource whose range
+        // ends at the buffer's fill: one 12-byte non-temporal load per lane,
+        // zeros past the end
+        const uint64_t base = reinterpret_cast<uint64_t>(a.cand) + 12ull * cstep;
+        const uint64_t left = n > cstep ? n - cstep : 0;
+        const uint32_t nrec = static_cast<uint32_t>(12ull * (left < 64 ? left : 64));
+        i32x4 rsrc;
+        rsrc.x = static_cast<int>(static_cast<uint32_t>(base));
+        rsrc.y = static_cast<int>(static_cast<uint32_t>(base >> 32) & 0xFFFFu);
+        rsrc.z = static_cast<int>(nrec);
+        rsrc.w = 0x00020000;
+        const i32x3 q = raw_buffer_load_v3(rsrc, lane * 12, 0, 2);
+        const uint32_t u0 = static_cast<uint32_t>(q.x), u1 = static_cast<uint32_t>(q.y),
+                       u2 = static_cast<uint32_t>(q.z);
         unpack12(u0, u1, u2, key_l);
         if (!(u2 >> 31)) key_l[0] = 0;
       } else {
